@@ -1,0 +1,134 @@
+/*
+ * tnx.h -- C ABI of libtnx, the MI355X (gfx950) block-sparse contraction engine.
+ *
+ * Plain pointers and sizes only.  Device buffers are owned by the caller
+ * (torch / any HIP allocator); the library owns plan objects and the device
+ * work lists / intermediate buffers inside them.  All device work is enqueued
+ * on the caller's hipStream_t (passed as void*).  Every entry point returns a
+ * status code (TNX_OK = 0, negative on failure); tnx_last_error() returns the
+ * message of the last failure on the calling thread.
+ *
+ * Packed block-sparse storage (one tensor = one contiguous float64 buffer):
+ * blocks in lexicographically sorted key order, each block row-major in its
+ * own leg order; offsets[i] is the element offset of block i.  This is the
+ * reference's canonical pytree order (src/tenax/core/tensor.py:572-578).
+ */
+#ifndef TNX_H
+#define TNX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define TNX_OK 0
+#define TNX_ERR_BAD_STRUCTURE (-1)
+#define TNX_ERR_CONSERVATION (-2)
+#define TNX_ERR_LABEL_COUNT (-3)
+#define TNX_ERR_DTYPE (-4)
+#define TNX_ERR_HIP (-5)
+#define TNX_ERR_OOM (-6)
+#define TNX_ERR_ARG (-7)
+#define TNX_ERR_CAPACITY (-8)
+
+/* arithmetic used by the block GEMMs; storage is always float64 */
+#define TNX_COMPUTE_F64 0       /* FP64 MFMA, FP64 accumulation            */
+#define TNX_COMPUTE_F32 1       /* FP32 MFMA, FP32 accumulation            */
+#define TNX_COMPUTE_SPLIT_F32 2 /* hi/lo FP32 split, 3 FP32 MFMA products,
+                                   FP32 per 16-wide K chunk, FP64 across   */
+
+/* planner modes */
+#define TNX_MODE_PAIRWISE_FILTERED 0 /* contractor.py:291-484 semantics        */
+#define TNX_MODE_MULTI_REACHABLE 1   /* dmrg.py:1048-1151 semantics (no filter) */
+
+/* One leg as the planner sees it: sorted distinct charges with their
+ * multiplicities (number of basis states carrying the charge).  Replaces the
+ * per-leg TensorIndex charge arrays (src/tenax/core/index.py:43-77). */
+typedef struct {
+  int32_t n_charges;
+  const int32_t *charges; /* host, ascending */
+  const int32_t *mult;    /* host, > 0 */
+  int32_t flow;           /* +1 IN, -1 OUT (index.py:27-40) */
+} tnx_leg;
+
+/* Block table of one packed tensor (host arrays). */
+typedef struct {
+  int32_t rank;
+  int32_t n_blocks;
+  const int32_t *keys;    /* n_blocks*rank, sorted lexicographically */
+  const int32_t *shapes;  /* n_blocks*rank */
+  const int64_t *offsets; /* n_blocks, element offsets */
+} tnx_blocks;
+
+typedef struct tnx_plan_s tnx_plan;
+
+int tnx_version(void);
+int tnx_last_error(char *buf, size_t n);
+int tnx_device_sync(void);
+
+/* Valid-sector enumeration on the device.
+ * Replaces tenax.core.tensor._compute_valid_blocks (src/tenax/core/tensor.py:53-165)
+ * and _block_slices shapes (tensor.py:168-188).  sym_n = 0 for U(1), n for Z_n.
+ * Keys come back sorted lexicographically (the order of tensor.py:576/651).
+ * If the count exceeds `capacity`, returns TNX_ERR_CAPACITY with *n_out set. */
+int tnx_valid_blocks(const tnx_leg *legs, int32_t rank, int32_t sym_n,
+                     int32_t has_target, int32_t target, int32_t capacity,
+                     int32_t *keys_out, int32_t *shapes_out, int32_t *n_out,
+                     void *stream);
+
+/* Pairwise block-sparse contraction plan, built on the device.
+ * Replaces the matching/filter/output-key stage of
+ * tenax.contraction.contractor._contract_symmetric (contractor.py:344-441)
+ * for two operands, and the two-operand case of dmrg._blockwise_contract
+ * (dmrg.py:1079-1136) in TNX_MODE_MULTI_REACHABLE.
+ * Operand layouts: A legs = [free..., contracted...] (a_kfirst=0) or
+ * [contracted..., free...] (a_kfirst=1); B legs = [contracted..., free...]
+ * (b_kfirst=1) or [free..., contracted...] (b_kfirst=0); the n_contracted
+ * contracted legs correspond in order.  Output legs = [free A..., free B...]
+ * described by out_legs. */
+int tnx_plan_pairwise(const tnx_blocks *a, const tnx_blocks *b, int32_t a_kfirst,
+                      int32_t b_kfirst, int32_t n_contracted,
+                      const tnx_leg *out_legs, int32_t sym_n, int32_t mode,
+                      int32_t has_target, int32_t target, void *stream,
+                      tnx_plan **plan);
+
+/* Output structure of a plan (pairwise / transpose plans). */
+int tnx_plan_output_info(const tnx_plan *plan, int32_t *n_blocks, int32_t *rank,
+                         int64_t *total_elems);
+int tnx_plan_output_copy(const tnx_plan *plan, int32_t *keys, int32_t *shapes,
+                         int64_t *offsets);
+/* Algorithmic work of one execute: flops (2*M*N*K summed over block GEMMs)
+ * and bytes (operands read once + output written once, float64). */
+int tnx_plan_stats(const tnx_plan *plan, double *flops, double *bytes,
+                   int32_t *n_pairs, int32_t *n_tasks);
+int tnx_plan_free(tnx_plan *plan);
+
+/* Grouped execution: every block pair of the plan in ONE launch (output tiles
+ * of uneven blocks, K-segments of all contributing pairs accumulated in a
+ * fixed order, no atomics).  Replaces the per-combination opt_einsum loop of
+ * contractor.py:444-476. */
+int tnx_contract_execute(const tnx_plan *plan, const double *a, const double *b,
+                         double *c, int32_t compute, void *stream);
+
+/* Plain per-block GPU path: one GEMM launch per block pair (the structure of
+ * contractor.py:444-476 on a GPU).  Same arithmetic; used as the comparison
+ * baseline. */
+int tnx_contract_execute_perblock(const tnx_plan *plan, const double *a,
+                                  const double *b, double *c, int32_t compute,
+                                  void *stream);
+
+/* Leg permutation of a packed tensor (SymmetricTensor.transpose,
+ * src/tenax/core/tensor.py:832-861): re-sorted keys on the device, one
+ * block-transpose launch. */
+int tnx_plan_transpose(const tnx_blocks *a, const tnx_leg *legs, const int32_t *perm,
+                       void *stream, tnx_plan **plan);
+int tnx_transpose_execute(const tnx_plan *plan, const double *a, double *out,
+                          void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TNX_H */

@@ -1,0 +1,390 @@
+"""Oracle: U(1) Heisenberg MPO, random symmetric MPS and two-site DMRG (FP64).
+
+Restates reference src/tenax/algorithms/auto_mpo.py:38-48, 93-386, 643-689
+(finite-automaton MPO and its U(1) bond charges), algorithms/dmrg.py:44-305
+(sweep schedule), 355-398 (environment lists), 808-882 (SVD + truncation),
+890-975 (trivial environments, boundary padding), 978-1045 (Lanczos),
+1165-1299 (environment updates, two-site update), 1373-1523 (builders).
+Test infrastructure only (see oracle/__init__.py).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from oracle.contract import blockwise_contract, contract
+from oracle.core import IN, OUT, U1, Leg, SymT, inner
+from oracle.linalg import scale_bond_axis, truncated_svd_symmetric
+
+SPIN_HALF = {
+    "Sz": np.array([[0.5, 0.0], [0.0, -0.5]]),
+    "Sp": np.array([[0.0, 1.0], [0.0, 0.0]]),
+    "Sm": np.array([[0.0, 0.0], [1.0, 0.0]]),
+    "Id": np.eye(2),
+}
+
+
+def _op_charge(op, pc):
+    """auto_mpo.py:245-277."""
+    q = None
+    for i in range(op.shape[0]):
+        for j in range(op.shape[1]):
+            if abs(op[i, j]) > 1e-14:
+                t = int(pc[i]) - int(pc[j])
+                if q is None:
+                    q = t
+                elif q != t:
+                    raise ValueError("Operator mixes charge sectors")
+    return 0 if q is None else q
+
+
+def build_auto_mpo_symmetric(terms_spec, L, phys_charges=(1, -1)):
+    """auto_mpo.py:93-205 (W matrices), 280-318 (bond charges), 321-386 (wrap).
+
+    terms_spec: tuples (coeff, op, site, op, site, ...).  Returns a list of
+    rank-4 SymT with legs (w_l IN, top IN, bot OUT, w_r OUT).
+    """
+    pc = np.asarray(phys_charges, dtype=np.int32)
+    d = 2
+    terms = []
+    for t in terms_spec:
+        pairs = sorted(((t[k + 1], SPIN_HALF[t[k]]) for k in range(1, len(t), 2)), key=lambda x: x[0])
+        terms.append((float(t[0]), tuple(pairs)))
+    bond_states = []
+    for j in range(L - 1):
+        st, idx = {}, 1
+        for tid, (_, ops) in enumerate(terms):
+            sites = [s for s, _ in ops]
+            if min(sites) <= j < max(sites):
+                st[tid] = idx
+                idx += 1
+        bond_states.append(st)
+    ident = np.eye(d)
+    ws = []
+    for i in range(L):
+        Dl = 1 if i == 0 else len(bond_states[i - 1]) + 2
+        Dr = 1 if i == L - 1 else len(bond_states[i]) + 2
+        W = np.zeros((Dl, d, d, Dr))
+        vac_l = 0 if i == 0 else Dl - 1
+        vac_r = Dr - 1
+        if L == 1:
+            pass
+        elif i == 0:
+            W[0, :, :, vac_r] = ident
+        elif i == L - 1:
+            W[0, :, :, 0] = ident
+        else:
+            W[0, :, :, 0] = ident
+            W[vac_l, :, :, vac_r] = ident
+        for tid, (c, ops) in enumerate(terms):
+            od = dict(ops)
+            sites = sorted(od)
+            mn, mx = sites[0], sites[-1]
+            if i < mn or i > mx:
+                continue
+            op = od.get(i, ident)
+            if mn == mx:
+                W[vac_l, :, :, 0] += c * op
+            elif i == mn:
+                W[vac_l, :, :, bond_states[i][tid]] += c * op
+            elif i == mx:
+                W[bond_states[i - 1][tid], :, :, 0] += op
+            else:
+                W[bond_states[i - 1][tid], :, :, bond_states[i][tid]] = op
+        ws.append(W)
+    bcs = []
+    for j in range(L - 1):
+        ch = np.zeros(len(bond_states[j]) + 2, dtype=np.int32)
+        for tid, si in bond_states[j].items():
+            ch[si] = sum(_op_charge(op, pc) for s, op in terms[tid][1] if s <= j)
+        bcs.append(ch)
+    sym = U1()
+    mpo = []
+    for i, W in enumerate(ws):
+        lc = np.zeros(W.shape[0], np.int32) if i == 0 else bcs[i - 1]
+        rc = np.zeros(W.shape[3], np.int32) if i == L - 1 else bcs[i]
+        ll = "w_left_0" if i == 0 else f"w{i - 1}_{i}"
+        rl = "w_right" if i == L - 1 else f"w{i}_{i + 1}"
+        legs = (Leg(sym, lc, IN, ll), Leg(sym, pc, IN, f"mpo_top_{i}"),
+                Leg(sym, pc, OUT, f"mpo_bot_{i}"), Leg(sym, rc, OUT, rl))
+        mpo.append(SymT.from_dense(W, legs))
+    return mpo
+
+
+def build_mpo_heisenberg(L, Jz=1.0, Jxy=1.0, hz=0.0):
+    """dmrg.py:1373-1412."""
+    terms = []
+    for i in range(L - 1):
+        terms.append((Jz, "Sz", i, "Sz", i + 1))
+        terms.append((Jxy / 2, "Sp", i, "Sm", i + 1))
+        terms.append((Jxy / 2, "Sm", i, "Sp", i + 1))
+    if hz != 0.0:
+        for i in range(L):
+            terms.append((hz, "Sz", i))
+    if not terms:
+        terms.append((0.0, "Sz", 0))
+    return build_auto_mpo_symmetric(terms, L)
+
+
+def mps_virtual_charges(bond_dim, target_charge=0):
+    """dmrg.py:1459-1479: charges per sector, padded with the middle charge."""
+    if target_charge == 0:
+        req = [-1, 0, 1]
+    else:
+        req = list(range(min(-1, target_charge - 1), max(1, target_charge + 1) + 1))
+    per = max(1, bond_dim // len(req))
+    v = np.concatenate([np.full(per, q, np.int32) for q in req])[:bond_dim]
+    if len(v) < bond_dim:
+        v = np.concatenate([v, np.full(bond_dim - len(v), req[len(req) // 2], np.int32)])
+    return v
+
+
+def build_random_symmetric_mps(L, bond_dim=4, seed=42, target_charge=0):
+    """dmrg.py:1415-1523 with the shared generator (site i seeded seed+i)."""
+    if target_charge % 2 != L % 2:
+        raise ValueError(f"target_charge={target_charge} has parity {target_charge % 2} "
+                         f"but L={L} has parity {L % 2}.")
+    sym = U1()
+    pc = np.array([1, -1], np.int32)
+    vc = mps_virtual_charges(bond_dim, target_charge)
+    sites = []
+    for i in range(L):
+        tgt = target_charge if i == L - 1 else None
+        if i == 0:
+            legs = (Leg(sym, pc, IN, f"p{i}"), Leg(sym, vc, OUT, f"v{i}_{i + 1}"))
+        elif i == L - 1:
+            legs = (Leg(sym, vc, IN, f"v{i - 1}_{i}"), Leg(sym, pc, IN, f"p{i}"))
+        else:
+            legs = (Leg(sym, vc, IN, f"v{i - 1}_{i}"), Leg(sym, pc, IN, f"p{i}"),
+                    Leg(sym, vc, OUT, f"v{i}_{i + 1}"))
+        sites.append(SymT.random_normal(legs, (seed + i,), target=tgt))
+    return sites
+
+
+# ----------------------------------------------------------- environments
+def trivial_left_env():
+    """dmrg.py:890-905."""
+    z = np.zeros(1, np.int32)
+    legs = (Leg(U1(), z, IN, "env_mps_l"), Leg(U1(), z, IN, "env_mpo_l"),
+            Leg(U1(), z, OUT, "env_mps_conj_l"))
+    return SymT({(0, 0, 0): np.ones((1, 1, 1))}, legs)
+
+
+def trivial_right_env():
+    """dmrg.py:908-921."""
+    z = np.zeros(1, np.int32)
+    legs = (Leg(U1(), z, OUT, "env_mps_r"), Leg(U1(), z, OUT, "env_mpo_r"),
+            Leg(U1(), z, IN, "env_mps_conj_r"))
+    return SymT({(0, 0, 0): np.ones((1, 1, 1))}, legs)
+
+
+def pad(t, pad_left):
+    """dmrg.py:924-952."""
+    z = np.zeros(1, np.int32)
+    if pad_left:
+        return SymT({(0,) + k: a[np.newaxis] for k, a in t.blocks.items()},
+                    (Leg(U1(), z, IN, "_pad_l"),) + t.legs, False)
+    return SymT({k + (0,): a[..., np.newaxis] for k, a in t.blocks.items()},
+                t.legs + (Leg(U1(), z, OUT, "_pad_r"),), False)
+
+
+def unpad(t, pad_left):
+    """dmrg.py:955-975."""
+    if pad_left:
+        return SymT({k[1:]: a[0] for k, a in t.blocks.items()}, t.legs[1:], False)
+    return SymT({k[:-1]: a[..., 0] for k, a in t.blocks.items()}, t.legs[:-1], False)
+
+
+def _pad_site(A):
+    if A.ndim == 2:
+        lab = A.labels()
+        return pad(A, isinstance(lab[0], str) and lab[0].startswith("p"))
+    return A
+
+
+def update_left_env(Lenv, A, W):
+    """dmrg.py:1165-1198: abc,apd,bpxe,cxf->def."""
+    A = _pad_site(A)
+    Ac = A.conj()
+    return blockwise_contract([Lenv, A, W, Ac], "abc,apd,bpxe,cxf->def",
+                              (A.legs[2], W.legs[3], Ac.legs[2]))
+
+
+def update_right_env(Renv, B, W):
+    """dmrg.py:1201-1232: abc,dpa,epxb,fxc->def."""
+    B = _pad_site(B)
+    Bc = B.conj()
+    return blockwise_contract([Renv, B, W, Bc], "abc,dpa,epxb,fxc->def",
+                              (B.legs[0], W.legs[0], Bc.legs[0]))
+
+
+def matvec(Lenv, v, Wl, Wr, Renv):
+    """dmrg.py:1282-1289: abc,apqd,bpse,eqtf,dfg->cstg."""
+    return blockwise_contract([Lenv, v, Wl, Wr, Renv], "abc,apqd,bpse,eqtf,dfg->cstg", v.legs)
+
+
+def lanczos(mv, initial, num_steps, tol, reorth=False):
+    """dmrg.py:978-1045.  ``reorth`` adds one classical Gram-Schmidt pass over
+    all previous basis vectors after the three-term recurrence (off in the
+    reference; the GPU path applies the same pass when enabled)."""
+    v = initial * (1.0 / (initial.norm() + 1e-15))
+    basis, alphas, betas = [v], [], [0.0]
+    for step in range(num_steps):
+        w = mv(basis[-1])
+        a = inner(basis[-1], w)
+        alphas.append(a)
+        w = w - basis[-1] * a
+        if step > 0:
+            w = w - basis[-2] * betas[-1]
+        if reorth:
+            cs = [inner(b, w) for b in basis]
+            for b, c in zip(basis, cs):
+                w = w - b * c
+        b = w.norm()
+        betas.append(b)
+        if b < tol:
+            break
+        basis.append(w * (1.0 / b))
+    n = len(alphas)
+    if n == 1:
+        return alphas[0], basis[0]
+    T = np.diag(alphas) + np.diag(betas[1:n], 1) + np.diag(betas[1:n], -1)
+    ev, evec = np.linalg.eigh(T)
+    i = int(np.argmin(ev))
+    c = evec[:, i]
+    x = basis[0] * float(c[0])
+    for k in range(1, n):
+        x = x + basis[k] * float(c[k])
+    x = x * (1.0 / (x.norm() + 1e-15))
+    return float(ev[i]), x
+
+
+@dataclass
+class DMRGConfig:
+    """dmrg.py:44-77 (noise is never read by the reference and is omitted)."""
+
+    max_bond_dim: int = 100
+    num_sweeps: int = 10
+    convergence_tol: float = 1e-10
+    lanczos_max_iter: int = 50
+    lanczos_tol: float = 1e-12
+    svd_trunc_err: float | None = None
+    lanczos_reorth: bool = False
+
+
+def two_site_update(sl, sr, Lenv, Wl, Wr, Renv, cfg):
+    """dmrg.py:1235-1299."""
+    theta = contract(sl, sr) if set(sl.labels()) & set(sr.labels()) else sl
+    nd = theta.ndim
+    pad_left = False
+    if nd == 3:
+        if not (isinstance(theta.labels()[0], str) and theta.labels()[0].startswith("v")):
+            pad_left = True
+            theta = pad(theta, True)
+        else:
+            theta = pad(theta, False)
+    e, t = lanczos(lambda v: matvec(Lenv, v, Wl, Wr, Renv), theta,
+                   cfg.lanczos_max_iter, cfg.lanczos_tol, cfg.lanczos_reorth)
+    if nd == 3:
+        t = unpad(t, pad_left)
+    return t, e
+
+
+def svd_and_truncate(theta, site, cfg, sweep_right=True):
+    """dmrg.py:808-882."""
+    lab = theta.labels()
+    lv = f"v{site - 1}_{site}" if site > 0 else None
+    rv = f"v{site + 1}_{site + 2}"
+    ll = [x for x in lab if x in (lv, f"p{site}") and x is not None]
+    rl = [x for x in lab if x in (rv, f"p{site + 1}") and x is not None]
+    if not ll or not rl:
+        n = len(lab)
+        ll, rl = list(lab[: n // 2]), list(lab[n // 2:])
+    bl = f"v{site}_{site + 1}"
+    A, s, B, s_full, _ = truncated_svd_symmetric(theta, ll, rl, bl, cfg.max_bond_dim, cfg.svd_trunc_err)
+    if len(s_full) > len(s):
+        err = float(np.sqrt(np.sum(s_full[len(s):] ** 2) / (np.sum(s_full ** 2) + 1e-15)))
+    else:
+        err = 0.0
+    if sweep_right:
+        B = scale_bond_axis(B, bl, s)
+    else:
+        A = scale_bond_axis(A, bl, s)
+    return A, s, B, err
+
+
+def dmrg(mpo, mps, cfg, on_update=None):
+    """dmrg.py:125-305 (two-site, symmetric backend).
+
+    Returns (energy, energies_per_sweep, mps, truncation_errors, converged).
+    ``on_update(sweep, i, direction, energy)`` is an optional trace hook.
+    """
+    L = len(mpo)
+    mps = list(mps)
+    left = [None] * (L + 1)
+    left[0] = trivial_left_env()
+    for i in range(L - 1):
+        left[i + 1] = update_left_env(left[i], mps[i], mpo[i])
+
+    def build_right():
+        r = [None] * (L + 1)
+        r[L] = trivial_right_env()
+        for i in range(L - 1, 0, -1):
+            r[i] = update_right_env(r[i + 1], mps[i], mpo[i])
+        return r
+
+    right = build_right()
+    energies, errs = [], []
+    energy, conv = 0.0, False
+    for sweep in range(cfg.num_sweeps):
+        prev = energy
+        if sweep > 0:
+            left = [None] * (L + 1)
+            left[0] = trivial_left_env()
+            for i in range(L - 1):
+                left[i + 1] = update_left_env(left[i], mps[i], mpo[i])
+        for i in range(L - 1):
+            r = right[i + 2] if right[i + 2] is not None else trivial_right_env()
+            th, energy = two_site_update(mps[i], mps[i + 1], left[i], mpo[i], mpo[i + 1], r, cfg)
+            A, s, B, err = svd_and_truncate(th, i, cfg)
+            mps[i], mps[i + 1] = A, B
+            errs.append(err)
+            left[i + 1] = update_left_env(left[i], A, mpo[i])
+            if on_update:
+                on_update(sweep, i, "R", energy)
+        right = build_right()
+        for i in range(L - 2, -1, -1):
+            r2 = right[i + 2] if right[i + 2] is not None else trivial_right_env()
+            th, energy = two_site_update(mps[i], mps[i + 1], left[i], mpo[i], mpo[i + 1], r2, cfg)
+            A, s, B, err = svd_and_truncate(th, i, cfg, sweep_right=False)
+            mps[i], mps[i + 1] = A, B
+            errs.append(err)
+            right[i + 1] = update_right_env(r2, B, mpo[i + 1])
+            if on_update:
+                on_update(sweep, i, "L", energy)
+        energies.append(energy)
+        if sweep > 0 and abs(energy - prev) < cfg.convergence_tol:
+            conv = True
+            break
+    return energy, energies, mps, errs, conv
+
+
+def heisenberg_ed_matrix(L, Jz=1.0, Jxy=1.0):
+    """Independent Kronecker-product Hamiltonian (not from the MPO path)."""
+    sz = SPIN_HALF["Sz"]
+    sp = SPIN_HALF["Sp"]
+    sm = SPIN_HALF["Sm"]
+
+    def site_op(op, i):
+        m = np.array([[1.0]])
+        for j in range(L):
+            m = np.kron(m, op if j == i else np.eye(2))
+        return m
+
+    H = np.zeros((2 ** L, 2 ** L))
+    for i in range(L - 1):
+        H += Jz * site_op(sz, i) @ site_op(sz, i + 1)
+        H += 0.5 * Jxy * (site_op(sp, i) @ site_op(sm, i + 1) + site_op(sm, i) @ site_op(sp, i + 1))
+    return H

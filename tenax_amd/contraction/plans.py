@@ -1,0 +1,89 @@
+"""Plan cache: device plans keyed by block structure.
+
+A plan depends only on structure (keys, shapes, layout, output legs), never on
+values, so one plan serves every call with the same structure -- e.g. every
+Lanczos iteration of a DMRG update.  Mirrors the role of the reference's
+shape-keyed expression caches (contractor.py:150-166, 405-457; dmrg.py:1099-1107).
+"""
+
+from __future__ import annotations
+
+import threading
+from collections import OrderedDict
+
+import numpy as np
+
+from tenax_amd import _lib
+
+_MAX_PLANS = 1024
+_cache: "OrderedDict[tuple, object]" = OrderedDict()
+_lock = threading.Lock()
+
+
+def _get(key, build):
+    with _lock:
+        hit = _cache.get(key)
+        if hit is not None:
+            _cache.move_to_end(key)
+            return hit
+    val = build()
+    with _lock:
+        _cache[key] = val
+        while len(_cache) > _MAX_PLANS:
+            _cache.popitem(last=False)
+    return val
+
+
+def clear_plan_cache():
+    with _lock:
+        _cache.clear()
+
+
+def _struct_sig(t):
+    return (t.keys.tobytes(), t.shapes.tobytes(), t.keys.shape)
+
+
+def _legs_sig(indices):
+    return tuple((ix.sectors()[0].tobytes(), ix.sectors()[1].tobytes(), int(ix.flow)) for ix in indices)
+
+
+def pairwise_plan(A, B, a_kfirst: int, b_kfirst: int, nc: int, out_indices, mode: int, target):
+    """Plan C = A . B (device planner); returns (Plan, keys, shapes, offsets, total)."""
+    sym = A.indices[0].symmetry if A.indices else out_indices[0].symmetry
+    key = ("pair", _struct_sig(A), _struct_sig(B), A.ndim, B.ndim, a_kfirst, b_kfirst, nc,
+           _legs_sig(out_indices), sym.sym_n, mode, target)
+
+    def build():
+        lib = _lib.load()
+        ta = _lib.BlockTable(A.keys, A.shapes, A.offsets, A.ndim)
+        tb = _lib.BlockTable(B.keys, B.shapes, B.offsets, B.ndim)
+        legs = _lib.LegArray([(*ix.sectors(), int(ix.flow)) for ix in out_indices])
+        ptr = _lib.C.c_void_p()
+        has_t = target is not None
+        rc = lib.tnx_plan_pairwise(_lib.C.byref(ta.s), _lib.C.byref(tb.s), a_kfirst, b_kfirst, nc, legs.arr,
+                                   sym.sym_n, mode, int(has_t), int(target) if has_t else 0,
+                                   _lib.stream_handle(), _lib.C.byref(ptr))
+        _lib.check(rc)
+        plan = _lib.Plan(ptr)
+        keys, shapes, offs, tot = plan.output()
+        return plan, keys, shapes, offs, tot
+
+    return _get(key, build)
+
+
+def transpose_plan(T, axes):
+    key = ("transpose", _struct_sig(T), T.ndim, tuple(axes), _legs_sig(T.indices))
+
+    def build():
+        lib = _lib.load()
+        ta = _lib.BlockTable(T.keys, T.shapes, T.offsets, T.ndim)
+        legs = _lib.LegArray([(*ix.sectors(), int(ix.flow)) for ix in T.indices])
+        perm = np.asarray(axes, np.int32)
+        ptr = _lib.C.c_void_p()
+        _lib.check(lib.tnx_plan_transpose(_lib.C.byref(ta.s), legs.arr, _lib.i32p(perm), _lib.stream_handle(),
+                                          _lib.C.byref(ptr)))
+        plan = _lib.Plan(ptr)
+        keys, shapes, offs, tot = plan.output()
+        return plan, keys, shapes, offs, tot
+
+    return _get(key, build)

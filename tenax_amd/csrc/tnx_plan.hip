@@ -1,0 +1,861 @@
+// Device-side integer planning for block-sparse contractions.
+//
+// Everything that decides the sector structure runs as GPU kernels:
+//  * valid-sector enumeration (tenax core/tensor.py:53-165): one thread per
+//    candidate charge tuple in mixed-radix order over each leg's sorted distinct
+//    charges, U(1) last leg solved from the running sum, then an order-preserving
+//    compaction -> keys come out lexicographically sorted;
+//  * block matching (contractor.py:375-441, dmrg.py:1111-1136): one thread per
+//    (A block, B block) candidate, equality of contracted-leg charges, output key
+//    located by binary search in the sorted valid set (conservation filter) or in
+//    the full candidate set (reachability semantics);
+//  * output offsets, pair grouping by output block (stable radix sort, so pairs
+//    accumulate in (A index, B index) order), GEMM tiles and their LPT order.
+// Only the finished tables are copied back (the Python layer needs keys/offsets).
+#include "tnx_internal.h"
+
+#include <rocprim/rocprim.hpp>
+
+#include <algorithm>
+#include <cstdio>
+#include <numeric>
+
+namespace tnx {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string &m) { g_last_error = m; }
+
+constexpr int MAX_RANK = 12;
+
+// ------------------------------------------------------------------ primitives
+template <class T>
+static void dev_exclusive_scan(const T *in, T *out, size_t n, hipStream_t s, DevBuf<char> &tmp) {
+  if (n == 0) return;
+  size_t tb = 0;
+  TNX_CHECK_HIP(rocprim::exclusive_scan(nullptr, tb, in, out, T(0), n, rocprim::plus<T>(), s));
+  tmp.alloc(tb ? tb : 1);
+  TNX_CHECK_HIP(rocprim::exclusive_scan(tmp.p, tb, in, out, T(0), n, rocprim::plus<T>(), s));
+}
+
+template <class K, class V>
+static void dev_sort_pairs(const K *kin, K *kout, const V *vin, V *vout, size_t n, int end_bit, hipStream_t s,
+                           DevBuf<char> &tmp) {
+  if (n == 0) return;
+  size_t tb = 0;
+  TNX_CHECK_HIP(rocprim::radix_sort_pairs(nullptr, tb, kin, kout, vin, vout, n, 0, end_bit, s));
+  tmp.alloc(tb ? tb : 1);
+  TNX_CHECK_HIP(rocprim::radix_sort_pairs(tmp.p, tb, kin, kout, vin, vout, n, 0, end_bit, s));
+}
+
+static int bits_for(uint64_t v) {
+  int b = 1;
+  while (b < 64 && (v >> b) != 0) ++b;
+  return b;
+}
+
+static unsigned nblk(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+// ------------------------------------------------------------------ legs on device
+struct LegDev {
+  int32_t off;   // into concatenated charge/mult arrays
+  int32_t n;     // distinct charges
+  int32_t flow;  // +1 / -1
+  int32_t pad;
+};
+
+struct LegTable {
+  std::vector<LegDev> h;
+  std::vector<int32_t> ch, mu;
+  DevBuf<LegDev> d_legs;
+  DevBuf<int32_t> d_ch, d_mu;
+  void build(const tnx_leg *legs, int rank, hipStream_t s) {
+    h.resize(rank);
+    ch.clear();
+    mu.clear();
+    for (int l = 0; l < rank; ++l) {
+      TNX_REQUIRE(legs[l].n_charges > 0, TNX_ERR_BAD_STRUCTURE, "leg with no charges");
+      TNX_REQUIRE(legs[l].flow == 1 || legs[l].flow == -1, TNX_ERR_ARG, "leg flow must be +1 or -1");
+      h[l] = LegDev{(int32_t)ch.size(), legs[l].n_charges, legs[l].flow, 0};
+      for (int c = 0; c < legs[l].n_charges; ++c) {
+        if (c) TNX_REQUIRE(legs[l].charges[c] > legs[l].charges[c - 1], TNX_ERR_ARG, "leg charges must be strictly ascending");
+        TNX_REQUIRE(legs[l].mult[c] > 0, TNX_ERR_ARG, "leg multiplicity must be positive");
+        ch.push_back(legs[l].charges[c]);
+        mu.push_back(legs[l].mult[c]);
+      }
+    }
+    d_legs.upload(h.data(), h.size(), s);
+    d_ch.upload(ch.data(), ch.size(), s);
+    d_mu.upload(mu.data(), mu.size(), s);
+  }
+};
+
+__device__ __forceinline__ int64_t pymod(int64_t a, int64_t n) {
+  int64_t r = a % n;
+  return r < 0 ? r + n : r;
+}
+
+__device__ __forceinline__ int find_charge(const int32_t *ch, int n, int32_t q) {
+  int lo = 0, hi = n - 1;
+  while (lo <= hi) {
+    int mid = (lo + hi) >> 1;
+    int32_t v = ch[mid];
+    if (v == q) return mid;
+    if (v < q) lo = mid + 1; else hi = mid - 1;
+  }
+  return -1;
+}
+
+// mode: 0 = conservation filter (U(1): last leg solved), 1 = all tuples.
+__global__ void enum_keys_kernel(int rank, const LegDev *__restrict__ legs, const int32_t *__restrict__ ch,
+                                 const int32_t *__restrict__ mu, int sym_n, int all_tuples, int64_t target,
+                                 int solve_last, int64_t ncand, int32_t *__restrict__ keys,
+                                 int32_t *__restrict__ shapes, int32_t *__restrict__ flag) {
+  int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= ncand) return;
+  const int nenum = solve_last ? rank - 1 : rank;
+  int32_t key[MAX_RANK], shp[MAX_RANK];
+  int64_t r = idx;
+  for (int l = nenum - 1; l >= 0; --l) {
+    const int n = legs[l].n;
+    const int d = (int)(r % n);
+    r /= n;
+    key[l] = ch[legs[l].off + d];
+    shp[l] = mu[legs[l].off + d];
+  }
+  int ok = 1;
+  if (!all_tuples) {
+    if (rank == 1) {
+      // tensor.py:89-95: single leg, raw flow*q == target
+      ok = ((int64_t)legs[0].flow * key[0] == target);
+    } else if (solve_last) {
+      // tensor.py:134-149: U(1) running sum, last charge determined
+      int64_t fused = 0;
+      for (int l = 0; l < nenum; ++l) fused += (int64_t)legs[l].flow * key[l];
+      const LegDev last = legs[rank - 1];
+      const int64_t ql = (target - fused) * last.flow;
+      int pos = -1;
+      if (ql >= INT32_MIN && ql <= INT32_MAX) pos = find_charge(ch + last.off, last.n, (int32_t)ql);
+      if (pos < 0) {
+        ok = 0;
+      } else {
+        key[rank - 1] = (int32_t)ql;
+        shp[rank - 1] = mu[last.off + pos];
+      }
+    } else {
+      // tensor.py:99-128,150-163: Z_n, first leg raw, every fuse reduced mod n
+      int64_t fused = (int64_t)legs[0].flow * key[0];
+      for (int l = 1; l < rank; ++l) fused = pymod(fused + (int64_t)legs[l].flow * key[l], sym_n);
+      ok = (fused == target);
+    }
+  }
+  flag[idx] = ok;
+  if (ok) {
+    for (int l = 0; l < rank; ++l) {
+      keys[idx * rank + l] = key[l];
+      shapes[idx * rank + l] = shp[l];
+    }
+  }
+}
+
+__global__ void compact_keys_kernel(int rank, int64_t ncand, const int32_t *__restrict__ flag,
+                                    const int64_t *__restrict__ pos, const int32_t *__restrict__ kin,
+                                    const int32_t *__restrict__ sin, int32_t *__restrict__ kout,
+                                    int32_t *__restrict__ sout) {
+  int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= ncand || !flag[idx]) return;
+  const int64_t o = pos[idx];
+  for (int l = 0; l < rank; ++l) {
+    kout[o * rank + l] = kin[idx * rank + l];
+    sout[o * rank + l] = sin[idx * rank + l];
+  }
+}
+
+__global__ void flag_to_i64_kernel(int64_t n, const int32_t *__restrict__ f, int64_t *__restrict__ o) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) o[i] = f[i];
+}
+
+// Sorted key set on the device.
+struct KeySet {
+  int32_t rank = 0;
+  int64_t n = 0;
+  DevBuf<int32_t> keys, shapes;
+};
+
+static constexpr int64_t MAX_CANDIDATES = int64_t(1) << 28;
+
+static void enumerate_keys(const tnx_leg *legs, int rank, int sym_n, bool all_tuples, bool has_target,
+                           int target, hipStream_t s, KeySet &out) {
+  TNX_REQUIRE(rank >= 1 && rank <= MAX_RANK, TNX_ERR_ARG, "rank out of range");
+  TNX_REQUIRE(sym_n >= 0, TNX_ERR_ARG, "sym_n must be >= 0");
+  LegTable lt;
+  lt.build(legs, rank, s);
+  const bool u1 = sym_n == 0;
+  const bool solve_last = !all_tuples && u1 && rank >= 2;
+  const int nenum = solve_last ? rank - 1 : rank;
+  int64_t ncand = 1;
+  for (int l = 0; l < nenum; ++l) {
+    ncand *= legs[l].n_charges;
+    TNX_REQUIRE(ncand <= MAX_CANDIDATES, TNX_ERR_CAPACITY, "sector enumeration exceeds 2^28 candidates");
+  }
+  const int64_t tgt = has_target ? target : 0;
+  DevBuf<int32_t> ckeys(ncand * rank), cshapes(ncand * rank), flag(ncand);
+  DevBuf<int64_t> f64(ncand), pos(ncand);
+  DevBuf<char> tmp;
+  hipLaunchKernelGGL(enum_keys_kernel, dim3(nblk(ncand)), dim3(256), 0, s, rank, lt.d_legs.p, lt.d_ch.p,
+                     lt.d_mu.p, sym_n, all_tuples ? 1 : 0, tgt, solve_last ? 1 : 0, ncand, ckeys.p, cshapes.p,
+                     flag.p);
+  TNX_CHECK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(flag_to_i64_kernel, dim3(nblk(ncand)), dim3(256), 0, s, ncand, flag.p, f64.p);
+  dev_exclusive_scan(f64.p, pos.p, ncand, s, tmp);
+  int64_t last_pos = 0, last_flag = 0;
+  TNX_CHECK_HIP(hipMemcpyAsync(&last_pos, pos.p + ncand - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  TNX_CHECK_HIP(hipMemcpyAsync(&last_flag, f64.p + ncand - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  TNX_CHECK_HIP(hipStreamSynchronize(s));
+  out.rank = rank;
+  out.n = last_pos + last_flag;
+  out.keys.alloc(std::max<int64_t>(out.n, 1) * rank);
+  out.shapes.alloc(std::max<int64_t>(out.n, 1) * rank);
+  if (out.n)
+    hipLaunchKernelGGL(compact_keys_kernel, dim3(nblk(ncand)), dim3(256), 0, s, rank, ncand, flag.p, pos.p,
+                       ckeys.p, cshapes.p, out.keys.p, out.shapes.p);
+  TNX_CHECK_HIP(hipGetLastError());
+  TNX_CHECK_HIP(hipStreamSynchronize(s));
+}
+
+// ------------------------------------------------------------------ pairwise planning
+__device__ __forceinline__ int key_cmp(const int32_t *a, const int32_t *b, int r) {
+  for (int l = 0; l < r; ++l) {
+    if (a[l] < b[l]) return -1;
+    if (a[l] > b[l]) return 1;
+  }
+  return 0;
+}
+
+__device__ __forceinline__ int64_t find_key(const int32_t *V, int64_t nV, const int32_t *k, int r) {
+  int64_t lo = 0, hi = nV - 1;
+  while (lo <= hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    const int c = key_cmp(V + mid * r, k, r);
+    if (c == 0) return mid;
+    if (c < 0) lo = mid + 1; else hi = mid - 1;
+  }
+  return -1;
+}
+
+struct PairSpec {
+  int32_t nA, nB, rA, rB, nc, rOut;
+  int32_t a_c0, b_c0;  // first contracted leg position in A / B
+  int32_t a_f0, b_f0;  // first free leg position in A / B
+};
+
+__global__ void match_pairs_kernel(PairSpec sp, const int32_t *__restrict__ kA, const int32_t *__restrict__ kB,
+                                   const int32_t *__restrict__ V, int64_t nV, int64_t *__restrict__ pair_out,
+                                   int32_t *__restrict__ present) {
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)sp.nA * sp.nB) return;
+  const int i = (int)(idx / sp.nB), j = (int)(idx % sp.nB);
+  const int32_t *ka = kA + (int64_t)i * sp.rA;
+  const int32_t *kb = kB + (int64_t)j * sp.rB;
+  int64_t res = -1;
+  bool ok = true;
+  for (int c = 0; c < sp.nc; ++c)
+    if (ka[sp.a_c0 + c] != kb[sp.b_c0 + c]) { ok = false; break; }
+  if (ok) {
+    int32_t ok_[MAX_RANK];
+    int w = 0;
+    for (int l = 0; l < sp.rA - sp.nc; ++l) ok_[w++] = ka[sp.a_f0 + l];
+    for (int l = 0; l < sp.rB - sp.nc; ++l) ok_[w++] = kb[sp.b_f0 + l];
+    res = find_key(V, nV, ok_, sp.rOut);
+    if (res >= 0) present[res] = 1;
+  }
+  pair_out[idx] = res;
+}
+
+__global__ void pair_flags_kernel(int64_t n, const int64_t *__restrict__ pair_out, int64_t *__restrict__ f) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) f[i] = pair_out[i] >= 0 ? 1 : 0;
+}
+
+__global__ void compact_pairs_kernel(int64_t n, const int64_t *__restrict__ pair_out,
+                                     const int64_t *__restrict__ pos, const int64_t *__restrict__ out_id,
+                                     uint32_t *__restrict__ vp_oid, uint32_t *__restrict__ vp_idx) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n || pair_out[i] < 0) return;
+  vp_oid[pos[i]] = (uint32_t)out_id[pair_out[i]];
+  vp_idx[pos[i]] = (uint32_t)i;
+}
+
+__global__ void present_to_i64_kernel(int64_t n, const int32_t *__restrict__ p, int64_t *__restrict__ o) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) o[i] = p[i] ? 1 : 0;
+}
+
+// Output block tables: keys, shapes, sizes and (M, N) split of each present key.
+__global__ void out_tables_kernel(int64_t nV, int rOut, int nfreeA, const int32_t *__restrict__ present,
+                                  const int64_t *__restrict__ out_id, const int32_t *__restrict__ V,
+                                  const int32_t *__restrict__ Vs, int32_t *__restrict__ okeys,
+                                  int32_t *__restrict__ oshapes, int64_t *__restrict__ osize,
+                                  int32_t *__restrict__ oM, int32_t *__restrict__ oN) {
+  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v >= nV || !present[v]) return;
+  const int64_t o = out_id[v];
+  int64_t M = 1, N = 1;
+  for (int l = 0; l < rOut; ++l) {
+    okeys[o * rOut + l] = V[v * rOut + l];
+    const int32_t d = Vs[v * rOut + l];
+    oshapes[o * rOut + l] = d;
+    if (l < nfreeA) M *= d; else N *= d;
+  }
+  osize[o] = M * N;
+  oM[o] = (int32_t)M;
+  oN[o] = (int32_t)N;
+}
+
+__global__ void segments_kernel(int64_t n, const uint32_t *__restrict__ soid, int32_t *__restrict__ pbeg,
+                                int32_t *__restrict__ pend) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t o = soid[k];
+  if (k == 0 || soid[k - 1] != o) pbeg[o] = (int32_t)k;
+  if (k == n - 1 || soid[k + 1] != o) pend[o] = (int32_t)(k + 1);
+}
+
+__global__ void build_pairs_kernel(int64_t n, PairSpec sp, int a_kfirst, int b_kfirst,
+                                   const uint32_t *__restrict__ sidx, const int32_t *__restrict__ sA,
+                                   const int32_t *__restrict__ sB, const int64_t *__restrict__ offA,
+                                   const int64_t *__restrict__ offB, GemmPair *__restrict__ pairs) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int64_t p = sidx[k];
+  const int i = (int)(p / sp.nB), j = (int)(p % sp.nB);
+  int64_t K = 1, M = 1, N = 1;
+  for (int c = 0; c < sp.nc; ++c) K *= sA[(int64_t)i * sp.rA + sp.a_c0 + c];
+  for (int l = 0; l < sp.rA - sp.nc; ++l) M *= sA[(int64_t)i * sp.rA + sp.a_f0 + l];
+  for (int l = 0; l < sp.rB - sp.nc; ++l) N *= sB[(int64_t)j * sp.rB + sp.b_f0 + l];
+  GemmPair g;
+  g.a_off = offA[i];
+  g.b_off = offB[j];
+  g.K = (int32_t)K;
+  g.lda = a_kfirst ? (int32_t)M : (int32_t)K;
+  g.ldb = b_kfirst ? (int32_t)N : (int32_t)K;
+  g.pad = 0;
+  pairs[k] = g;
+}
+
+__global__ void build_outs_kernel(int64_t nout, const int64_t *__restrict__ ooff, const int32_t *__restrict__ oM,
+                                  const int32_t *__restrict__ oN, const int32_t *__restrict__ pbeg,
+                                  const int32_t *__restrict__ pend, GemmOut *__restrict__ outs) {
+  const int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (o >= nout) return;
+  GemmOut g;
+  g.c_off = ooff[o];
+  g.M = oM[o];
+  g.N = oN[o];
+  g.ldc = oN[o];
+  g.pair_begin = pbeg[o];
+  g.pair_end = pend[o];
+  g.pad = 0;
+  outs[o] = g;
+}
+
+// ------------------------------------------------------------------ tiling (device)
+__global__ void tile_count_kernel(int64_t nout, const GemmOut *__restrict__ outs, const GemmPair *__restrict__ pairs,
+                                  int64_t *__restrict__ ntile, uint64_t *__restrict__ costkey, uint32_t *__restrict__ ids,
+                                  uint64_t maxcost) {
+  const int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (o >= nout) return;
+  const GemmOut g = outs[o];
+  ntile[o] = (int64_t)((g.M + GEMM_BM - 1) / GEMM_BM) * ((g.N + GEMM_BN - 1) / GEMM_BN);
+  uint64_t cost = 0;
+  for (int k = g.pair_begin; k < g.pair_end; ++k) cost += (uint64_t)((pairs[k].K + GEMM_BK - 1) / GEMM_BK);
+  if (cost > maxcost) cost = maxcost;
+  costkey[o] = maxcost - cost;  // ascending sort -> longest first
+  ids[o] = (uint32_t)o;
+}
+
+__global__ void gather_i64_kernel(int64_t n, const uint32_t *__restrict__ idx, const int64_t *__restrict__ in,
+                                  int64_t *__restrict__ out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[idx[i]];
+}
+
+__global__ void write_tasks_kernel(int64_t nout, const uint32_t *__restrict__ order, const int64_t *__restrict__ toff,
+                                   const GemmOut *__restrict__ outs, GemmTask *__restrict__ tasks) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= nout) return;
+  const int o = (int)order[r];
+  const GemmOut g = outs[o];
+  const int tn = (g.N + GEMM_BN - 1) / GEMM_BN;
+  const int tm = (g.M + GEMM_BM - 1) / GEMM_BM;
+  int64_t t = toff[r];
+  for (int a = 0; a < tm; ++a)
+    for (int b = 0; b < tn; ++b) tasks[t++] = GemmTask{o, a * GEMM_BM, b * GEMM_BN, 0};
+}
+
+void finalize_program(GemmProgram &prog, hipStream_t s) {
+  const int64_t nout = prog.n_outs;
+  prog.n_tasks = 0;
+  if (nout == 0) {
+    prog.h_outs.clear();
+    prog.h_pairs.clear();
+    prog.flops = 0;
+    return;
+  }
+  DevBuf<int64_t> ntile(nout), ntile_sorted(nout), toff(nout);
+  DevBuf<uint64_t> ck(nout), ck_sorted(nout);
+  DevBuf<uint32_t> ids(nout), order(nout);
+  DevBuf<char> tmp;
+  const uint64_t maxcost = (uint64_t(1) << 40) - 1;
+  hipLaunchKernelGGL(tile_count_kernel, dim3(nblk(nout)), dim3(256), 0, s, nout, prog.outs.p, prog.pairs.p, ntile.p,
+                     ck.p, ids.p, maxcost);
+  TNX_CHECK_HIP(hipGetLastError());
+  dev_sort_pairs(ck.p, ck_sorted.p, ids.p, order.p, nout, 40, s, tmp);
+  hipLaunchKernelGGL(gather_i64_kernel, dim3(nblk(nout)), dim3(256), 0, s, nout, order.p, ntile.p, ntile_sorted.p);
+  dev_exclusive_scan(ntile_sorted.p, toff.p, nout, s, tmp);
+  int64_t last_off = 0, last_n = 0;
+  TNX_CHECK_HIP(hipMemcpyAsync(&last_off, toff.p + nout - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  TNX_CHECK_HIP(hipMemcpyAsync(&last_n, ntile_sorted.p + nout - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  TNX_CHECK_HIP(hipStreamSynchronize(s));
+  const int64_t ntasks = last_off + last_n;
+  TNX_REQUIRE(ntasks < (int64_t(1) << 31), TNX_ERR_CAPACITY, "too many GEMM tiles");
+  prog.tasks.alloc(std::max<int64_t>(ntasks, 1));
+  hipLaunchKernelGGL(write_tasks_kernel, dim3(nblk(nout)), dim3(256), 0, s, nout, order.p, toff.p, prog.outs.p,
+                     prog.tasks.p);
+  TNX_CHECK_HIP(hipGetLastError());
+  prog.n_tasks = (int32_t)ntasks;
+  prog.h_outs.resize(prog.n_outs);
+  prog.h_pairs.resize(prog.n_pairs);
+  prog.outs.download(prog.h_outs.data(), prog.n_outs, s);
+  prog.pairs.download(prog.h_pairs.data(), prog.n_pairs, s);
+  TNX_CHECK_HIP(hipStreamSynchronize(s));
+  double f = 0;
+  for (const auto &o : prog.h_outs)
+    for (int k = o.pair_begin; k < o.pair_end; ++k) f += 2.0 * o.M * (double)o.N * prog.h_pairs[k].K;
+  prog.flops = f;
+}
+
+// ------------------------------------------------------------------ plan objects
+struct PairwisePlan : public tnx_plan_s {
+  GemmProgram prog;
+  // per-block path: one single-pair program per pair (tasks contiguous per pair)
+  DevBuf<GemmOut> pb_outs;
+  DevBuf<GemmTask> pb_tasks;
+  std::vector<int32_t> pb_task_off, pb_task_cnt;
+  std::vector<uint8_t> pb_first;
+  bool pb_ready = false;
+  int64_t a_total = 0, b_total = 0;
+};
+
+struct TDesc {
+  int64_t in_off, out_off, size;
+  int32_t rank, pad;
+  int32_t shape[MAX_RANK];
+  int64_t stride[MAX_RANK];  // input stride of output dim d
+};
+
+struct TransposePlan : public tnx_plan_s {
+  DevBuf<TDesc> desc;
+  DevBuf<int32_t> task_blk;
+  DevBuf<int64_t> task_start;
+  int32_t n_tasks_t = 0;
+};
+
+static void build_pairwise(PairwisePlan &P, const tnx_blocks *a, const tnx_blocks *b, int a_kfirst, int b_kfirst,
+                           int nc, const tnx_leg *out_legs, int sym_n, int mode, bool has_target, int target,
+                           hipStream_t s) {
+  TNX_REQUIRE(a && b && out_legs, TNX_ERR_ARG, "null argument");
+  TNX_REQUIRE(nc >= 0 && nc <= a->rank && nc <= b->rank, TNX_ERR_ARG, "bad contracted leg count");
+  const int rOut = a->rank + b->rank - 2 * nc;
+  TNX_REQUIRE(rOut >= 1 && rOut <= MAX_RANK, TNX_ERR_ARG, "output rank out of range");
+  TNX_REQUIRE(mode == TNX_MODE_PAIRWISE_FILTERED || mode == TNX_MODE_MULTI_REACHABLE, TNX_ERR_ARG, "bad mode");
+  P.kind = PlanKind::Pairwise;
+  P.out_rank = rOut;
+  P.prog.a_lay = a_kfirst ? LAY_KM : LAY_MK;
+  P.prog.b_lay = b_kfirst ? LAY_KN : LAY_NK;
+  for (int i = 0; i < a->n_blocks; ++i) {
+    int64_t sz = 1;
+    for (int l = 0; l < a->rank; ++l) sz *= a->shapes[i * a->rank + l];
+    P.a_total = std::max(P.a_total, a->offsets[i] + sz);
+  }
+  for (int j = 0; j < b->n_blocks; ++j) {
+    int64_t sz = 1;
+    for (int l = 0; l < b->rank; ++l) sz *= b->shapes[j * b->rank + l];
+    P.b_total = std::max(P.b_total, b->offsets[j] + sz);
+  }
+
+  KeySet V;
+  enumerate_keys(out_legs, rOut, sym_n, mode == TNX_MODE_MULTI_REACHABLE, has_target, target, s, V);
+
+  PairSpec sp;
+  sp.nA = a->n_blocks;
+  sp.nB = b->n_blocks;
+  sp.rA = a->rank;
+  sp.rB = b->rank;
+  sp.nc = nc;
+  sp.rOut = rOut;
+  sp.a_c0 = a_kfirst ? 0 : a->rank - nc;
+  sp.a_f0 = a_kfirst ? nc : 0;
+  sp.b_c0 = b_kfirst ? 0 : b->rank - nc;
+  sp.b_f0 = b_kfirst ? nc : 0;
+  const int64_t npc = (int64_t)sp.nA * sp.nB;
+  TNX_REQUIRE(npc < (int64_t(1) << 32), TNX_ERR_CAPACITY, "too many candidate block pairs");
+
+  P.out_nblocks = 0;
+  P.out_total = 0;
+  P.out_keys.clear();
+  P.out_shapes.clear();
+  P.out_offsets.clear();
+  if (npc == 0 || V.n == 0) {
+    P.prog.n_outs = 0;
+    P.prog.n_pairs = 0;
+    finalize_program(P.prog, s);
+    return;
+  }
+  DevBuf<int32_t> kA, kB, sA, sB;
+  DevBuf<int64_t> oA, oB;
+  kA.upload(a->keys, (size_t)sp.nA * sp.rA, s);
+  kB.upload(b->keys, (size_t)sp.nB * sp.rB, s);
+  sA.upload(a->shapes, (size_t)sp.nA * sp.rA, s);
+  sB.upload(b->shapes, (size_t)sp.nB * sp.rB, s);
+  oA.upload(a->offsets, sp.nA, s);
+  oB.upload(b->offsets, sp.nB, s);
+
+  DevBuf<int64_t> pair_out(npc), pflag(npc), ppos(npc);
+  DevBuf<int32_t> present(V.n);
+  DevBuf<int64_t> present64(V.n), out_id(V.n);
+  DevBuf<char> tmp;
+  TNX_CHECK_HIP(hipMemsetAsync(present.p, 0, V.n * sizeof(int32_t), s));
+  hipLaunchKernelGGL(match_pairs_kernel, dim3(nblk(npc)), dim3(256), 0, s, sp, kA.p, kB.p, V.keys.p, V.n,
+                     pair_out.p, present.p);
+  TNX_CHECK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(pair_flags_kernel, dim3(nblk(npc)), dim3(256), 0, s, npc, pair_out.p, pflag.p);
+  dev_exclusive_scan(pflag.p, ppos.p, npc, s, tmp);
+  hipLaunchKernelGGL(present_to_i64_kernel, dim3(nblk(V.n)), dim3(256), 0, s, V.n, present.p, present64.p);
+  dev_exclusive_scan(present64.p, out_id.p, V.n, s, tmp);
+  int64_t h4[4];
+  TNX_CHECK_HIP(hipMemcpyAsync(&h4[0], ppos.p + npc - 1, 8, hipMemcpyDeviceToHost, s));
+  TNX_CHECK_HIP(hipMemcpyAsync(&h4[1], pflag.p + npc - 1, 8, hipMemcpyDeviceToHost, s));
+  TNX_CHECK_HIP(hipMemcpyAsync(&h4[2], out_id.p + V.n - 1, 8, hipMemcpyDeviceToHost, s));
+  TNX_CHECK_HIP(hipMemcpyAsync(&h4[3], present64.p + V.n - 1, 8, hipMemcpyDeviceToHost, s));
+  TNX_CHECK_HIP(hipStreamSynchronize(s));
+  const int64_t npairs = h4[0] + h4[1];
+  const int64_t nout = h4[2] + h4[3];
+  TNX_REQUIRE(npairs < (int64_t(1) << 31) && nout < (int64_t(1) << 31), TNX_ERR_CAPACITY, "plan too large");
+  P.prog.n_pairs = (int32_t)npairs;
+  P.prog.n_outs = (int32_t)nout;
+  P.out_nblocks = (int32_t)nout;
+  if (nout == 0) {
+    finalize_program(P.prog, s);
+    return;
+  }
+  DevBuf<uint32_t> vp_oid(npairs), vp_idx(npairs), s_oid(npairs), s_idx(npairs);
+  hipLaunchKernelGGL(compact_pairs_kernel, dim3(nblk(npc)), dim3(256), 0, s, npc, pair_out.p, ppos.p, out_id.p,
+                     vp_oid.p, vp_idx.p);
+  TNX_CHECK_HIP(hipGetLastError());
+  dev_sort_pairs(vp_oid.p, s_oid.p, vp_idx.p, s_idx.p, npairs, bits_for((uint64_t)nout), s, tmp);
+
+  DevBuf<int32_t> okeys(nout * rOut), oshapes(nout * rOut), oM(nout), oN(nout), pbeg(nout), pend(nout);
+  DevBuf<int64_t> osize(nout), ooff(nout);
+  hipLaunchKernelGGL(out_tables_kernel, dim3(nblk(V.n)), dim3(256), 0, s, V.n, rOut, a->rank - nc, present.p,
+                     out_id.p, V.keys.p, V.shapes.p, okeys.p, oshapes.p, osize.p, oM.p, oN.p);
+  dev_exclusive_scan(osize.p, ooff.p, nout, s, tmp);
+  hipLaunchKernelGGL(segments_kernel, dim3(nblk(npairs)), dim3(256), 0, s, npairs, s_oid.p, pbeg.p, pend.p);
+  P.prog.pairs.alloc(npairs);
+  P.prog.outs.alloc(nout);
+  hipLaunchKernelGGL(build_pairs_kernel, dim3(nblk(npairs)), dim3(256), 0, s, npairs, sp, a_kfirst, b_kfirst,
+                     s_idx.p, sA.p, sB.p, oA.p, oB.p, P.prog.pairs.p);
+  hipLaunchKernelGGL(build_outs_kernel, dim3(nblk(nout)), dim3(256), 0, s, nout, ooff.p, oM.p, oN.p, pbeg.p,
+                     pend.p, P.prog.outs.p);
+  TNX_CHECK_HIP(hipGetLastError());
+  finalize_program(P.prog, s);
+
+  P.out_keys.resize(nout * rOut);
+  P.out_shapes.resize(nout * rOut);
+  P.out_offsets.resize(nout);
+  okeys.download(P.out_keys.data(), nout * rOut, s);
+  oshapes.download(P.out_shapes.data(), nout * rOut, s);
+  ooff.download(P.out_offsets.data(), nout, s);
+  TNX_CHECK_HIP(hipStreamSynchronize(s));
+  int64_t total = 0;
+  for (int64_t o = 0; o < nout; ++o) {
+    int64_t sz = 1;
+    for (int l = 0; l < rOut; ++l) sz *= P.out_shapes[o * rOut + l];
+    total = std::max(total, P.out_offsets[o] + sz);
+  }
+  P.out_total = total;
+  P.flops = P.prog.flops;
+  P.bytes = 8.0 * (double)(P.a_total + P.b_total + P.out_total);
+  P.n_pairs = P.prog.n_pairs;
+  P.n_tasks = P.prog.n_tasks;
+}
+
+static void ensure_perblock(PairwisePlan &P, hipStream_t s) {
+  if (P.pb_ready) return;
+  const auto &ho = P.prog.h_outs;
+  const auto &hp = P.prog.h_pairs;
+  std::vector<GemmOut> outs(hp.size());
+  std::vector<GemmTask> tasks;
+  P.pb_task_off.assign(hp.size(), 0);
+  P.pb_task_cnt.assign(hp.size(), 0);
+  P.pb_first.assign(hp.size(), 0);
+  for (size_t o = 0; o < ho.size(); ++o) {
+    for (int k = ho[o].pair_begin; k < ho[o].pair_end; ++k) {
+      GemmOut g = ho[o];
+      g.pair_begin = k;
+      g.pair_end = k + 1;
+      outs[k] = g;
+      P.pb_first[k] = (k == ho[o].pair_begin);
+      P.pb_task_off[k] = (int32_t)tasks.size();
+      for (int m0 = 0; m0 < g.M; m0 += GEMM_BM)
+        for (int n0 = 0; n0 < g.N; n0 += GEMM_BN) tasks.push_back(GemmTask{k, m0, n0, 0});
+      P.pb_task_cnt[k] = (int32_t)tasks.size() - P.pb_task_off[k];
+    }
+  }
+  P.pb_outs.upload(outs.data(), outs.size(), s);
+  P.pb_tasks.upload(tasks.data(), tasks.size(), s);
+  TNX_CHECK_HIP(hipStreamSynchronize(s));
+  P.pb_ready = true;
+}
+
+// ------------------------------------------------------------------ transpose
+__global__ void transpose_code_kernel(int nb, int rank, const int32_t *__restrict__ keys,
+                                      const int32_t *__restrict__ perm, const LegDev *__restrict__ legs,
+                                      const int32_t *__restrict__ ch, uint64_t *__restrict__ code,
+                                      uint32_t *__restrict__ ids) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  uint64_t c = 0;
+  for (int d = 0; d < rank; ++d) {
+    const int l = perm[d];
+    const LegDev L = legs[l];
+    const int pos = find_charge(ch + L.off, L.n, keys[(int64_t)i * rank + l]);
+    c = c * (uint64_t)L.n + (uint64_t)(pos < 0 ? 0 : pos);
+  }
+  code[i] = c;
+  ids[i] = (uint32_t)i;
+}
+
+__global__ void transpose_kernel(const TDesc *__restrict__ desc, const int32_t *__restrict__ tblk,
+                                 const int64_t *__restrict__ tstart, const double *__restrict__ in,
+                                 double *__restrict__ out) {
+  const TDesc d = desc[tblk[blockIdx.x]];
+  const int64_t e = tstart[blockIdx.x] + threadIdx.x;
+  if (e >= d.size) return;
+  int64_t r = e, src = 0;
+  for (int k = d.rank - 1; k >= 0; --k) {
+    const int64_t ix = r % d.shape[k];
+    r /= d.shape[k];
+    src += ix * d.stride[k];
+  }
+  out[d.out_off + e] = in[d.in_off + src];
+}
+
+static void build_transpose(TransposePlan &P, const tnx_blocks *a, const tnx_leg *legs, const int32_t *perm,
+                            hipStream_t s) {
+  TNX_REQUIRE(a && legs && perm, TNX_ERR_ARG, "null argument");
+  const int r = a->rank, nb = a->n_blocks;
+  TNX_REQUIRE(r >= 1 && r <= MAX_RANK, TNX_ERR_ARG, "rank out of range");
+  std::vector<int> seen(r, 0);
+  uint64_t space = 1;
+  for (int d = 0; d < r; ++d) {
+    TNX_REQUIRE(perm[d] >= 0 && perm[d] < r && !seen[perm[d]], TNX_ERR_ARG, "perm is not a permutation");
+    seen[perm[d]] = 1;
+    space *= (uint64_t)legs[perm[d]].n_charges;
+    TNX_REQUIRE(space < (uint64_t(1) << 62), TNX_ERR_CAPACITY, "key space too large");
+  }
+  P.kind = PlanKind::Transpose;
+  P.out_rank = r;
+  P.out_nblocks = nb;
+  if (nb == 0) return;
+  LegTable lt;
+  lt.build(legs, r, s);
+  DevBuf<int32_t> kA, dperm;
+  kA.upload(a->keys, (size_t)nb * r, s);
+  std::vector<int32_t> hp(perm, perm + r);
+  dperm.upload(hp.data(), r, s);
+  DevBuf<uint64_t> code(nb), code_s(nb);
+  DevBuf<uint32_t> ids(nb), order(nb);
+  DevBuf<char> tmp;
+  hipLaunchKernelGGL(transpose_code_kernel, dim3(nblk(nb)), dim3(256), 0, s, nb, r, kA.p, dperm.p, lt.d_legs.p,
+                     lt.d_ch.p, code.p, ids.p);
+  TNX_CHECK_HIP(hipGetLastError());
+  dev_sort_pairs(code.p, code_s.p, ids.p, order.p, nb, bits_for(space), s, tmp);
+  std::vector<uint32_t> h_order(nb);
+  order.download(h_order.data(), nb, s);
+  TNX_CHECK_HIP(hipStreamSynchronize(s));
+  // output tables in sorted order; per-block descriptors
+  P.out_keys.resize((size_t)nb * r);
+  P.out_shapes.resize((size_t)nb * r);
+  P.out_offsets.resize(nb);
+  std::vector<TDesc> desc(nb);
+  std::vector<int32_t> tblk;
+  std::vector<int64_t> tstart;
+  int64_t off = 0;
+  for (int k = 0; k < nb; ++k) {
+    const int i = (int)h_order[k];
+    TDesc d{};
+    d.rank = r;
+    d.in_off = a->offsets[i];
+    d.out_off = off;
+    int64_t st[MAX_RANK];
+    int64_t acc = 1;
+    for (int l = r - 1; l >= 0; --l) {
+      st[l] = acc;
+      acc *= a->shapes[(int64_t)i * r + l];
+    }
+    for (int dd = 0; dd < r; ++dd) {
+      P.out_keys[(size_t)k * r + dd] = a->keys[(int64_t)i * r + perm[dd]];
+      P.out_shapes[(size_t)k * r + dd] = a->shapes[(int64_t)i * r + perm[dd]];
+      d.shape[dd] = a->shapes[(int64_t)i * r + perm[dd]];
+      d.stride[dd] = st[perm[dd]];
+    }
+    d.size = acc;
+    P.out_offsets[k] = off;
+    desc[k] = d;
+    for (int64_t e = 0; e < acc; e += 256) {
+      tblk.push_back(k);
+      tstart.push_back(e);
+    }
+    off += acc;
+  }
+  P.out_total = off;
+  P.bytes = 16.0 * off;
+  P.desc.upload(desc.data(), nb, s);
+  P.task_blk.upload(tblk.data(), tblk.size(), s);
+  P.task_start.upload(tstart.data(), tstart.size(), s);
+  P.n_tasks_t = (int32_t)tblk.size();
+  P.n_tasks = P.n_tasks_t;
+  TNX_CHECK_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace tnx
+
+using namespace tnx;
+
+extern "C" {
+
+int tnx_version(void) { return 1000; }
+
+int tnx_last_error(char *buf, size_t n) {
+  if (!buf || n == 0) return TNX_ERR_ARG;
+  std::snprintf(buf, n, "%s", g_last_error.c_str());
+  return TNX_OK;
+}
+
+int tnx_device_sync(void) {
+  return guarded([&] { TNX_CHECK_HIP(hipDeviceSynchronize()); });
+}
+
+int tnx_valid_blocks(const tnx_leg *legs, int32_t rank, int32_t sym_n, int32_t has_target, int32_t target,
+                     int32_t capacity, int32_t *keys_out, int32_t *shapes_out, int32_t *n_out, void *stream) {
+  return guarded([&] {
+    TNX_REQUIRE(legs && n_out, TNX_ERR_ARG, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    KeySet V;
+    enumerate_keys(legs, rank, sym_n, false, has_target != 0, target, s, V);
+    TNX_REQUIRE(V.n < (int64_t(1) << 31), TNX_ERR_CAPACITY, "too many sectors");
+    *n_out = (int32_t)V.n;
+    if (V.n > capacity) throw Error{TNX_ERR_CAPACITY, "valid block count exceeds capacity"};
+    if (V.n) {
+      TNX_REQUIRE(keys_out && shapes_out, TNX_ERR_ARG, "null output");
+      V.keys.download(keys_out, V.n * rank, s);
+      V.shapes.download(shapes_out, V.n * rank, s);
+      TNX_CHECK_HIP(hipStreamSynchronize(s));
+    }
+  });
+}
+
+int tnx_plan_pairwise(const tnx_blocks *a, const tnx_blocks *b, int32_t a_kfirst, int32_t b_kfirst,
+                      int32_t n_contracted, const tnx_leg *out_legs, int32_t sym_n, int32_t mode,
+                      int32_t has_target, int32_t target, void *stream, tnx_plan **plan) {
+  return guarded([&] {
+    TNX_REQUIRE(plan, TNX_ERR_ARG, "null plan pointer");
+    std::unique_ptr<PairwisePlan> P(new PairwisePlan());
+    build_pairwise(*P, a, b, a_kfirst, b_kfirst, n_contracted, out_legs, sym_n, mode, has_target != 0, target,
+                   (hipStream_t)stream);
+    *plan = P.release();
+  });
+}
+
+int tnx_plan_output_info(const tnx_plan *plan, int32_t *n_blocks, int32_t *rank, int64_t *total_elems) {
+  return guarded([&] {
+    TNX_REQUIRE(plan, TNX_ERR_ARG, "null plan");
+    if (n_blocks) *n_blocks = plan->out_nblocks;
+    if (rank) *rank = plan->out_rank;
+    if (total_elems) *total_elems = plan->out_total;
+  });
+}
+
+int tnx_plan_output_copy(const tnx_plan *plan, int32_t *keys, int32_t *shapes, int64_t *offsets) {
+  return guarded([&] {
+    TNX_REQUIRE(plan, TNX_ERR_ARG, "null plan");
+    const size_t n = (size_t)plan->out_nblocks, r = (size_t)plan->out_rank;
+    if (keys && n) std::memcpy(keys, plan->out_keys.data(), n * r * sizeof(int32_t));
+    if (shapes && n) std::memcpy(shapes, plan->out_shapes.data(), n * r * sizeof(int32_t));
+    if (offsets && n) std::memcpy(offsets, plan->out_offsets.data(), n * sizeof(int64_t));
+  });
+}
+
+int tnx_plan_stats(const tnx_plan *plan, double *flops, double *bytes, int32_t *n_pairs, int32_t *n_tasks) {
+  return guarded([&] {
+    TNX_REQUIRE(plan, TNX_ERR_ARG, "null plan");
+    if (flops) *flops = plan->flops;
+    if (bytes) *bytes = plan->bytes;
+    if (n_pairs) *n_pairs = plan->n_pairs;
+    if (n_tasks) *n_tasks = plan->n_tasks;
+  });
+}
+
+int tnx_plan_free(tnx_plan *plan) {
+  return guarded([&] { delete plan; });
+}
+
+int tnx_contract_execute(const tnx_plan *plan, const double *a, const double *b, double *c, int32_t compute,
+                         void *stream) {
+  return guarded([&] {
+    TNX_REQUIRE(plan && plan->kind == PlanKind::Pairwise, TNX_ERR_ARG, "not a pairwise plan");
+    const PairwisePlan &P = static_cast<const PairwisePlan &>(*plan);
+    if (P.prog.n_tasks == 0) return;
+    TNX_REQUIRE(a && b && c, TNX_ERR_ARG, "null buffer");
+    launch_grouped_gemm(P.prog, a, b, c, compute, false, (hipStream_t)stream);
+  });
+}
+
+int tnx_contract_execute_perblock(const tnx_plan *plan, const double *a, const double *b, double *c,
+                                  int32_t compute, void *stream) {
+  return guarded([&] {
+    TNX_REQUIRE(plan && plan->kind == PlanKind::Pairwise, TNX_ERR_ARG, "not a pairwise plan");
+    PairwisePlan &P = const_cast<PairwisePlan &>(static_cast<const PairwisePlan &>(*plan));
+    hipStream_t s = (hipStream_t)stream;
+    ensure_perblock(P, s);
+    TNX_REQUIRE(a && b && c || P.prog.n_pairs == 0, TNX_ERR_ARG, "null buffer");
+    for (int k = 0; k < P.prog.n_pairs; ++k)
+      launch_prepared(P.prog.a_lay, P.prog.b_lay, compute, P.pb_task_cnt[k], s, P.pb_tasks.p + P.pb_task_off[k],
+                      P.pb_outs.p, P.prog.pairs.p, a, b, c, !P.pb_first[k]);
+  });
+}
+
+int tnx_plan_transpose(const tnx_blocks *a, const tnx_leg *legs, const int32_t *perm, void *stream,
+                       tnx_plan **plan) {
+  return guarded([&] {
+    TNX_REQUIRE(plan, TNX_ERR_ARG, "null plan pointer");
+    std::unique_ptr<TransposePlan> P(new TransposePlan());
+    build_transpose(*P, a, legs, perm, (hipStream_t)stream);
+    *plan = P.release();
+  });
+}
+
+int tnx_transpose_execute(const tnx_plan *plan, const double *a, double *out, void *stream) {
+  return guarded([&] {
+    TNX_REQUIRE(plan && plan->kind == PlanKind::Transpose, TNX_ERR_ARG, "not a transpose plan");
+    const TransposePlan &P = static_cast<const TransposePlan &>(*plan);
+    if (P.n_tasks_t == 0) return;
+    TNX_REQUIRE(a && out, TNX_ERR_ARG, "null buffer");
+    hipLaunchKernelGGL(transpose_kernel, dim3(P.n_tasks_t), dim3(256), 0, (hipStream_t)stream, P.desc.p,
+                       P.task_blk.p, P.task_start.p, a, out);
+    TNX_CHECK_HIP(hipGetLastError());
+  });
+}
+
+}  // extern "C"

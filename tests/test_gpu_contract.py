@@ -1,0 +1,191 @@
+"""GPU parity of the device planner and the grouped block GEMM against the oracle."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import contract as ocn
+from oracle import core as oc
+from tests.helpers import assert_same_structure, from_oracle, max_rel_err, oracle_legs, product_indices, to_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _legs(sym, rng, rank, maxq=2, dim=6):
+    out = []
+    for i in range(rank):
+        if sym.n is None:
+            ch = rng.integers(-maxq, maxq + 1, size=dim)
+        else:
+            ch = rng.integers(0, sym.n, size=dim)
+        out.append(oc.Leg(sym, ch, oc.IN if rng.random() < 0.5 else oc.OUT, f"l{i}"))
+    return tuple(out)
+
+
+@pytest.mark.parametrize("sym", [oc.U1(), oc.Zn(2), oc.Zn(3)])
+@pytest.mark.parametrize("rank", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("target", [None, 1])
+def test_valid_blocks_device_matches_oracle(gpu, sym, rank, target):
+    from tenax_amd.core.tensor import valid_blocks
+
+    rng = np.random.default_rng(100 * rank + (sym.n or 0))
+    legs = _legs(sym, rng, rank)
+    keys, shapes = valid_blocks(product_indices(legs), target)
+    ref = sorted(oc.compute_valid_blocks(legs, target))
+    assert [tuple(int(q) for q in k) for k in keys] == ref
+    for k, s in zip(keys, shapes):
+        _, shp = oc.block_slices(legs, tuple(k))
+        assert tuple(int(x) for x in s) == shp
+
+
+def c2_legs(chi):
+    from tenax_amd.workloads import PHYS_CHARGES, gaussian_bond_charges
+
+    bond = gaussian_bond_charges(chi)
+    U1 = oc.U1()
+    A = (oc.Leg(U1, bond, oc.IN, "l"), oc.Leg(U1, PHYS_CHARGES, oc.IN, "p"), oc.Leg(U1, PHYS_CHARGES, oc.IN, "pp"),
+         oc.Leg(U1, bond, oc.OUT, "r"))
+    B = (oc.Leg(U1, bond, oc.IN, "r"), oc.Leg(U1, PHYS_CHARGES, oc.IN, "q"), oc.Leg(U1, PHYS_CHARGES, oc.IN, "qq"),
+         oc.Leg(U1, bond, oc.OUT, "s"))
+    return A, B
+
+
+@pytest.fixture(scope="module")
+def c2_pair():
+    A, B = c2_legs(96)
+    oA = oc.SymT.random_normal(A, (1, 0))
+    oB = oc.SymT.random_normal(B, (1, 1))
+    oC, _ = ocn.contract_symmetric([oA, oB], "lpmr,rqns->lpmqns")
+    return oA, oB, oC
+
+
+def test_contract_c2_f64_matches_oracle(gpu, c2_pair):
+    from tenax_amd import contract
+
+    oA, oB, oC = c2_pair
+    C = contract(from_oracle(oA), from_oracle(oB))
+    assert C.labels() == ("l", "p", "pp", "q", "qq", "s")
+    assert_same_structure(C, oC)
+    assert max_rel_err(C, oC) < 1e-13
+
+
+@pytest.mark.parametrize("mode,tol", [("f32", 2e-5), ("split_f32", 2e-6)])
+def test_contract_c2_reduced_precision(gpu, c2_pair, mode, tol):
+    from tenax_amd import compute_mode, contract
+
+    oA, oB, oC = c2_pair
+    with compute_mode(mode):
+        C = contract(from_oracle(oA), from_oracle(oB))
+    assert_same_structure(C, oC)
+    err = max_rel_err(C, oC)
+    assert err < tol, err
+
+
+def test_perblock_path_matches_grouped(gpu, c2_pair):
+    from tenax_amd import contract, per_block_execution
+
+    oA, oB, oC = c2_pair
+    A, B = from_oracle(oA), from_oracle(oB)
+    C1 = contract(A, B)
+    with per_block_execution():
+        C2 = contract(A, B)
+    assert np.array_equal(C1.keys, C2.keys)
+    d = (C1.data - C2.data).abs().max().item()
+    assert d <= 1e-12 * C1.data.abs().max().item()
+
+
+def test_contract_needs_transposes_and_swap(gpu):
+    """Operands whose legs interleave: device transposes + swapped roles."""
+    from tenax_amd import contract
+
+    U1 = oc.U1()
+    rng = np.random.default_rng(5)
+    b1 = rng.integers(-2, 3, 7)
+    b2 = rng.integers(-1, 2, 5)
+    x = rng.integers(-1, 2, 4)
+    A = (oc.Leg(U1, b1, oc.IN, "i"), oc.Leg(U1, x, oc.OUT, "x"), oc.Leg(U1, b2, oc.IN, "j"),
+         oc.Leg(U1, rng.integers(-2, 3, 6), oc.OUT, "k"))
+    B = (oc.Leg(U1, b2, oc.OUT, "j"), oc.Leg(U1, rng.integers(-1, 2, 3), oc.IN, "y"), oc.Leg(U1, b1, oc.OUT, "i"),
+         oc.Leg(U1, rng.integers(-1, 2, 5), oc.IN, "z"))
+    oA = oc.SymT.random_normal(A, (9,))
+    oB = oc.SymT.random_normal(B, (10,))
+    for out in [None, ("z", "x", "y", "k"), ("x", "z", "k", "y")]:
+        C = contract(from_oracle(oA), from_oracle(oB), output_labels=out)
+        lab = C.labels()
+        sub = "ixjk,jyiz->" + "".join({"x": "x", "k": "k", "y": "y", "z": "z"}[l] for l in lab)
+        ref = np.einsum(sub, oA.todense(), oB.todense())
+        np.testing.assert_allclose(C.todense().cpu().numpy(), ref, atol=1e-12)
+
+
+@pytest.mark.parametrize("n", [2, 3])
+def test_contract_zn(gpu, n):
+    from tenax_amd import contract
+
+    Z = oc.Zn(n)
+    rng = np.random.default_rng(n)
+    a = rng.integers(0, n, 6)
+    b = rng.integers(0, n, 5)
+    A = (oc.Leg(Z, a, oc.IN, "a"), oc.Leg(Z, rng.integers(0, n, 4), oc.IN, "p"), oc.Leg(Z, b, oc.OUT, "b"))
+    B = (oc.Leg(Z, b, oc.IN, "b"), oc.Leg(Z, rng.integers(0, n, 3), oc.OUT, "c"))
+    oA = oc.SymT.random_normal(A, (3,))
+    oB = oc.SymT.random_normal(B, (4,))
+    oC, _ = ocn.contract_symmetric([oA, oB], "apb,bc->apc")
+    C = contract(from_oracle(oA), from_oracle(oB))
+    assert_same_structure(C, oC)
+    assert max_rel_err(C, oC) < 1e-13
+
+
+def test_three_operand_contract_matches_dense(gpu):
+    from tenax_amd import contract
+
+    U1 = oc.U1()
+    rng = np.random.default_rng(11)
+    c1, c2, c3 = (rng.integers(-2, 3, 5) for _ in range(3))
+    A = (oc.Leg(U1, c1, oc.IN, "a"), oc.Leg(U1, c2, oc.OUT, "b"))
+    B = (oc.Leg(U1, c2, oc.IN, "b"), oc.Leg(U1, c3, oc.OUT, "c"))
+    Cc = (oc.Leg(U1, c3, oc.IN, "c"), oc.Leg(U1, c1, oc.OUT, "d"))
+    ts = [oc.SymT.random_normal(l, (i,)) for i, l in enumerate((A, B, Cc))]
+    R = contract(*[from_oracle(t) for t in ts])
+    ref = np.einsum("ab,bc,cd->ad", *[t.todense() for t in ts])
+    np.testing.assert_allclose(R.todense().cpu().numpy(), ref, atol=1e-12)
+
+
+def test_transpose_matches_oracle(gpu, c2_pair):
+    oA, _, _ = c2_pair
+    A = from_oracle(oA)
+    for axes in [(3, 0, 1, 2), (1, 3, 0, 2), (0, 1, 2, 3)]:
+        T = A.transpose(axes)
+        oT = oA.transpose(axes)
+        assert_same_structure(T, oT)
+        assert max_rel_err(T, oT) == 0.0
+
+
+def test_from_dense_roundtrip_and_error(gpu):
+    from tenax_amd import SymmetricTensor
+
+    U1 = oc.U1()
+    legs = (oc.Leg(U1, [-1, 0, 1, 0], oc.IN, "a"), oc.Leg(U1, [1, -1], oc.IN, "p"),
+            oc.Leg(U1, [0, 1, -1, 2, 0], oc.OUT, "b"))
+    o = oc.SymT.random_normal(legs, (2,))
+    d = o.todense()
+    T = SymmetricTensor.from_dense(d, product_indices(legs))
+    assert_same_structure(T, o)
+    np.testing.assert_array_equal(T.todense().cpu().numpy(), d)
+    bad = d.copy()
+    bad[0, 0, 0] = 1.0  # (q=-1, p=+1, q=0): violates conservation
+    with pytest.raises(ValueError, match="outside symmetry-allowed"):
+        SymmetricTensor.from_dense(bad, product_indices(legs))
+
+
+def test_zeros_and_random_normal_structure(gpu):
+    from tenax_amd import PRNGKey, SymmetricTensor
+
+    legs = c2_legs(48)[0]
+    idx = product_indices(legs)
+    Z = SymmetricTensor.zeros(idx)
+    R = SymmetricTensor.random_normal(idx, PRNGKey(1))
+    o = oc.SymT.random_normal(legs, (1,))
+    assert_same_structure(Z, o)
+    assert_same_structure(R, o)
+    assert max_rel_err(R, o) == 0.0
+    assert float(Z.norm()) == 0.0
