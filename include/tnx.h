@@ -85,10 +85,10 @@ int tnx_valid_blocks(const tnx_leg *legs, int32_t rank, int32_t sym_n,
  * for two operands, and the two-operand case of dmrg._blockwise_contract
  * (dmrg.py:1079-1136) in TNX_MODE_MULTI_REACHABLE.
  * Operand layouts: A legs = [free..., contracted...] (a_kfirst=0) or
- * [contracted..., free...] (a_kfirst=1); B legs = [contracted..., free...]
- * (b_kfirst=1) or [free..., contracted...] (b_kfirst=0); the n_contracted
- * contracted legs correspond in order.  Output legs = [free A..., free B...]
- * described by out_legs. */
+ * [contracted..., free...] (a_kfirst=1); B legs must be
+ * [contracted..., free...] (b_kfirst=1; other values are rejected -- callers
+ * transpose B first); the n_contracted contracted legs correspond in order.
+ * Output legs = [free A..., free B...] described by out_legs. */
 int tnx_plan_pairwise(const tnx_blocks *a, const tnx_blocks *b, int32_t a_kfirst,
                       int32_t b_kfirst, int32_t n_contracted,
                       const tnx_leg *out_legs, int32_t sym_n, int32_t mode,

@@ -1,0 +1,88 @@
+/*
+ * tnx_dmrg.h -- DMRG contractions of libtnx (C ABI).
+ *
+ * The two-site effective-Hamiltonian matvec, the environment updates and the
+ * Lanczos eigensolver of the reference's symmetric DMRG backend, as device
+ * plans.  Conventions as in tnx.h (caller-owned float64 device buffers,
+ * caller stream, status codes).  Block tables use the packed layout of tnx.h.
+ *
+ * Factored evaluation (same sums as the reference's 5-/4-block products):
+ *   matvec:  X[(b,c),d] = sum_a L[a,(b,c)] th[a,d]            (grouped GEMM)
+ *            Y[c,(d,f)] = sum_b C[b,f] X[(b,c),d],
+ *                         C[b,f] = sum_e Wl[b,p,s,e] Wr[e,q,t,f]  (mix)
+ *            out[c,g]  += sum_(d,f) Y[c,(d,f)] R[(d,f),g]     (grouped GEMM)
+ *   env:     the same three stages with one MPO tensor.
+ * Requires physical legs of multiplicity one (distinct physical charges).
+ */
+#ifndef TNX_DMRG_H
+#define TNX_DMRG_H
+
+#include "tnx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Plan of out[c,s,t,g] = L[a,b,c] th[a,p,q,d] Wl[b,p,s,e] Wr[e,q,t,f] R[d,f,g]
+ * ("abc,apqd,bpse,eqtf,dfg->cstg", src/tenax/algorithms/dmrg.py:1282-1289,
+ * evaluated by _blockwise_contract, dmrg.py:1048-1151: equality matching of
+ * shared charges, no conservation filter).  `theta` is the input AND output
+ * block table (the full sector layout of the two-site tensor); output blocks
+ * that no block combination reaches are written as zeros. */
+int tnx_matvec_plan(const tnx_blocks *L, const tnx_blocks *theta, const tnx_blocks *Wl,
+                    const tnx_blocks *Wr, const tnx_blocks *R, void *stream, tnx_plan **plan);
+
+/* Reachability of the matvec plan: n_terms (theta block -> output block) edges. */
+int tnx_matvec_reach(const tnx_plan *plan, int32_t *n_edges, int32_t *src, int32_t *dst,
+                     int32_t capacity);
+
+/* y = H_eff x for the planned structure. */
+int tnx_matvec_execute(const tnx_plan *plan, const double *L, const double *Wl, const double *Wr,
+                       const double *R, const double *x, double *y, int32_t compute, void *stream);
+
+/* One stage of the fused matvec alone (profiling/roofline): stage 0 = MPO
+ * coefficient kernel, 1 = L.theta grouped GEMM, 2 = coefficient mix,
+ * 3 = .R grouped GEMM.  Stages read the intermediates the previous stage left
+ * in the plan, so stage k is meaningful after stages 0..k-1 ran once. */
+int tnx_matvec_execute_stage(const tnx_plan *plan, int32_t stage, const double *L,
+                             const double *Wl, const double *Wr, const double *R,
+                             const double *x, double *y, int32_t compute, void *stream);
+
+/* Algorithmic flops of each stage of a fused plan (array of 4: coef, GEMM1, mix, GEMM2). */
+int tnx_fused_stage_flops(const tnx_plan *plan, double *flops4);
+
+/* Plain per-block GPU path of the same matvec: one GEMM launch per block
+ * product of both stages (the reference's per-combination structure on a GPU). */
+int tnx_matvec_execute_perblock(const tnx_plan *plan, const double *L, const double *Wl,
+                                const double *Wr, const double *R, const double *x, double *y,
+                                int32_t compute, void *stream);
+
+/* Environment update plan:
+ *   new[d,e,f] = E[a,b,c] A[a,p,d] W[b,p,x,e] A[c,x,f]
+ *   ("abc,apd,bpxe,cxf->def", the left update of dmrg.py:1165-1198).
+ * The right update of dmrg.py:1201-1232 ("abc,dpa,epxb,fxc->def") is the same
+ * sum with the site transposed to (a,p,d) and the MPO tensor to (b,p,x,e); the
+ * caller passes those transposes.  `side` must be 0.  The MPS site A is used
+ * for ket and bra (real data, so A.conj() == A).  out_legs: the three output
+ * legs (candidate-key space); output structure (reachable keys, sorted) via
+ * tnx_plan_output_*. */
+int tnx_env_plan(int32_t side, const tnx_blocks *E, const tnx_blocks *A, const tnx_blocks *W,
+                 const tnx_leg *out_legs, void *stream, tnx_plan **plan);
+int tnx_env_execute(const tnx_plan *plan, const double *E, const double *A, const double *W,
+                    double *out, int32_t compute, void *stream);
+
+/* Lanczos ground state of the planned matvec (dmrg.py:978-1045), entirely on
+ * the device: FP64 alpha/beta and vector updates, optional one-pass classical
+ * Gram-Schmidt re-orthogonalisation against every previous Krylov vector,
+ * stop when beta < tol (later steps become no-ops on the device), lowest
+ * eigenpair of the tridiagonal matrix by Sturm bisection + inverse iteration
+ * on the device, Ritz vector sum_k c_k v_k normalised.  theta is overwritten
+ * with the eigenvector; *energy and *n_steps are returned to the host. */
+int tnx_lanczos(const tnx_plan *matvec_plan, const double *L, const double *Wl, const double *Wr,
+                const double *R, double *theta, int32_t max_iter, double tol, int32_t reorth,
+                int32_t compute, double *energy, int32_t *n_steps, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TNX_DMRG_H */

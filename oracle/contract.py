@@ -106,10 +106,24 @@ def contract_symmetric(tensors, subscripts):
             okey = tuple(cc.get(c, 0) for c in outp)
             if okey not in valid:
                 continue
-            r = np.einsum(subscripts, *[a for _, a in combo], optimize=True)
+            r = _einsum(subscripts, [a for _, a in combo])
             combos.append((okey, tuple(k for k, _ in combo)))
             out[okey] = out[okey] + r if okey in out else r
     return SymT(out, out_legs, validate=target is None), combos
+
+
+_PATHS: dict = {}
+
+
+def _einsum(subscripts, arrs):
+    """np.einsum with the contraction path cached by block shapes (the role of
+    the reference's shape-keyed opt_einsum expression cache, dmrg.py:1099-1107)."""
+    key = (subscripts, tuple(a.shape for a in arrs))
+    path = _PATHS.get(key)
+    if path is None:
+        path = np.einsum_path(subscripts, *arrs, optimize="optimal")[0]
+        _PATHS[key] = path
+    return np.einsum(subscripts, *arrs, optimize=path)
 
 
 def blockwise_contract(tensors, subscripts, out_legs, cache=None):
@@ -124,7 +138,7 @@ def blockwise_contract(tensors, subscripts, out_legs, cache=None):
     def rec(i):
         if i == len(tensors):
             okey = tuple(cc.get(c, 0) for c in outp)
-            acc.setdefault(okey, []).append(np.einsum(subscripts, *arrs, optimize=True))
+            acc.setdefault(okey, []).append(_einsum(subscripts, arrs))
             return
         s = isubs[i]
         for k, a in items[i]:
@@ -154,6 +168,59 @@ def blockwise_contract(tensors, subscripts, out_legs, cache=None):
             tot = tot + a
         out[k] = tot
     return SymT(out, out_legs, validate=False)
+
+
+def blockwise_keys(key_lists, subscripts):
+    """Output key set of dmrg.py:1094-1136's recursion, on keys only."""
+    inp, outp = subscripts.split("->")
+    isubs = inp.split(",")
+    out = set()
+    cc = {}
+
+    def rec(i):
+        if i == len(key_lists):
+            out.add(tuple(cc.get(c, 0) for c in outp))
+            return
+        for k in key_lists[i]:
+            added = []
+            ok = True
+            for c, q in zip(isubs[i], k):
+                q = int(q)
+                if c in cc:
+                    if cc[c] != q:
+                        ok = False
+                        break
+                else:
+                    cc[c] = q
+                    added.append(c)
+            if ok:
+                rec(i + 1)
+            for c in added:
+                del cc[c]
+
+    rec(0)
+    return out
+
+
+def extract_blocks(dense, legs, keys):
+    """Blocks of a dense array at the given keys (tensor.py:700-713 gather)."""
+    from oracle.core import block_slices
+
+    out = {}
+    for k in keys:
+        masks, shape = block_slices(legs, k)
+        if all(s > 0 for s in shape):
+            out[k] = dense[np.ix_(*[np.where(m)[0] for m in masks])].copy()
+    return out
+
+
+def blockwise_contract_dense(tensors, subscripts, out_legs):
+    """Same result as blockwise_contract: output keys from the key-level
+    recursion, values from one dense einsum of the densified operands (the
+    sum over block combinations restricted to those keys)."""
+    keys = blockwise_keys([list(t.blocks) for t in tensors], subscripts)
+    dense = _einsum(subscripts, [t.todense() for t in tensors])
+    return SymT(extract_blocks(dense, out_legs, keys), out_legs, validate=False)
 
 
 def contract(*tensors, output_labels=None):

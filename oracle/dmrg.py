@@ -14,7 +14,8 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from oracle.contract import blockwise_contract, contract
+from oracle.contract import (_einsum, blockwise_contract, blockwise_contract_dense, blockwise_keys, contract,
+                             extract_blocks)
 from oracle.core import IN, OUT, U1, Leg, SymT, inner
 from oracle.linalg import scale_bond_axis, truncated_svd_symmetric
 
@@ -204,25 +205,63 @@ def _pad_site(A):
     return A
 
 
+# Evaluation mode of the multi-operand contractions: "blocks" runs the
+# reference's per-combination loop; "dense" gives the same keys (key-level
+# recursion) with values from one dense einsum (fast for tests/fixtures; the
+# two are compared in tests/test_oracle.py).
+MODE = {"eval": "blocks"}
+
+
+def _bc(tensors, subs, legs):
+    if MODE["eval"] == "dense":
+        return blockwise_contract_dense(tensors, subs, legs)
+    return blockwise_contract(tensors, subs, legs)
+
+
 def update_left_env(Lenv, A, W):
     """dmrg.py:1165-1198: abc,apd,bpxe,cxf->def."""
     A = _pad_site(A)
     Ac = A.conj()
-    return blockwise_contract([Lenv, A, W, Ac], "abc,apd,bpxe,cxf->def",
-                              (A.legs[2], W.legs[3], Ac.legs[2]))
+    return _bc([Lenv, A, W, Ac], "abc,apd,bpxe,cxf->def", (A.legs[2], W.legs[3], Ac.legs[2]))
 
 
 def update_right_env(Renv, B, W):
     """dmrg.py:1201-1232: abc,dpa,epxb,fxc->def."""
     B = _pad_site(B)
     Bc = B.conj()
-    return blockwise_contract([Renv, B, W, Bc], "abc,dpa,epxb,fxc->def",
-                              (B.legs[0], W.legs[0], Bc.legs[0]))
+    return _bc([Renv, B, W, Bc], "abc,dpa,epxb,fxc->def", (B.legs[0], W.legs[0], Bc.legs[0]))
+
+
+MATVEC_SUBS = "abc,apqd,bpse,eqtf,dfg->cstg"
 
 
 def matvec(Lenv, v, Wl, Wr, Renv):
     """dmrg.py:1282-1289: abc,apqd,bpse,eqtf,dfg->cstg."""
-    return blockwise_contract([Lenv, v, Wl, Wr, Renv], "abc,apqd,bpse,eqtf,dfg->cstg", v.legs)
+    return _bc([Lenv, v, Wl, Wr, Renv], MATVEC_SUBS, v.legs)
+
+
+class DenseMatvec:
+    """matvec() with the environments densified once per two-site update."""
+
+    def __init__(self, Lenv, Wl, Wr, Renv):
+        self.ops = (Lenv, Wl, Wr, Renv)
+        self.d = [t.todense() for t in self.ops]
+        self._keys = {}
+
+    def __call__(self, v):
+        kv = tuple(sorted(v.blocks))
+        keys = self._keys.get(kv)
+        if keys is None:
+            L, Wl, Wr, R = self.ops
+            keys = blockwise_keys([list(L.blocks), list(kv), list(Wl.blocks), list(Wr.blocks), list(R.blocks)],
+                                  MATVEC_SUBS)
+            self._keys[kv] = keys
+        L, Wl, Wr, R = self.d
+        t = np.tensordot(L, v.todense(), axes=([0], [0]))    # b c p q d
+        t = np.tensordot(t, Wl, axes=([0, 2], [0, 1]))       # c q d s e
+        t = np.tensordot(t, Wr, axes=([1, 4], [1, 0]))       # c d s t f
+        dense = np.tensordot(t, R, axes=([1, 4], [0, 1]))    # c s t g
+        return SymT(extract_blocks(dense, v.legs, keys), v.legs, validate=False)
 
 
 def lanczos(mv, initial, num_steps, tol, reorth=False):
@@ -285,8 +324,9 @@ def two_site_update(sl, sr, Lenv, Wl, Wr, Renv, cfg):
             theta = pad(theta, True)
         else:
             theta = pad(theta, False)
-    e, t = lanczos(lambda v: matvec(Lenv, v, Wl, Wr, Renv), theta,
-                   cfg.lanczos_max_iter, cfg.lanczos_tol, cfg.lanczos_reorth)
+    mv = DenseMatvec(Lenv, Wl, Wr, Renv) if MODE["eval"] == "dense" else (
+        lambda v: matvec(Lenv, v, Wl, Wr, Renv))
+    e, t = lanczos(mv, theta, cfg.lanczos_max_iter, cfg.lanczos_tol, cfg.lanczos_reorth)
     if nd == 3:
         t = unpad(t, pad_left)
     return t, e

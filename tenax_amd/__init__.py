@@ -35,3 +35,17 @@ from tenax_amd.core import (
 )
 
 __version__ = "0.1.0"
+from tenax_amd.algorithms import (  # noqa: E402
+    AutoMPO,
+    DMRGConfig,
+    DMRGResult,
+    SweepOps,
+    build_auto_mpo,
+    build_mpo_heisenberg,
+    build_random_symmetric_mps,
+    compute_mps_sector,
+    dmrg,
+    validate_mps_sector,
+)
+from tenax_amd.linalg import svd, truncated_svd  # noqa: E402
+from tenax_amd.network import TensorNetwork  # noqa: E402

@@ -83,6 +83,17 @@ SIGNATURES = {
     "tnx_contract_execute_perblock": (C.c_int, [_p, _p, _p, _p, _i32, _p]),
     "tnx_plan_transpose": (C.c_int, [C.POINTER(Blocks), C.POINTER(Leg), _PI32, _p, C.POINTER(_p)]),
     "tnx_transpose_execute": (C.c_int, [_p, _p, _p, _p]),
+    # include/tnx_dmrg.h
+    "tnx_matvec_plan": (C.c_int, [C.POINTER(Blocks)] * 5 + [_p, C.POINTER(_p)]),
+    "tnx_matvec_reach": (C.c_int, [_p, _PI32, _PI32, _PI32, _i32]),
+    "tnx_matvec_execute": (C.c_int, [_p, _p, _p, _p, _p, _p, _p, _i32, _p]),
+    "tnx_matvec_execute_perblock": (C.c_int, [_p, _p, _p, _p, _p, _p, _p, _i32, _p]),
+    "tnx_matvec_execute_stage": (C.c_int, [_p, _i32, _p, _p, _p, _p, _p, _p, _i32, _p]),
+    "tnx_fused_stage_flops": (C.c_int, [_p, _PD]),
+    "tnx_env_plan": (C.c_int, [_i32, C.POINTER(Blocks), C.POINTER(Blocks), C.POINTER(Blocks), C.POINTER(Leg), _p,
+                               C.POINTER(_p)]),
+    "tnx_env_execute": (C.c_int, [_p, _p, _p, _p, _p, _i32, _p]),
+    "tnx_lanczos": (C.c_int, [_p, _p, _p, _p, _p, _p, _i32, C.c_double, _i32, _i32, _PD, _PI32, _p]),
 }
 
 _lib = None

@@ -1,0 +1,17 @@
+"""Algorithms on the accelerated path (mirrors reference src/tenax/algorithms)."""
+
+from tenax_amd.algorithms.auto_mpo import AutoMPO, build_auto_mpo, spin_half_ops, spin_one_ops
+from tenax_amd.algorithms.dmrg import (
+    DMRGConfig,
+    DMRGResult,
+    SweepOps,
+    build_mpo_heisenberg,
+    build_random_symmetric_mps,
+    compute_mps_sector,
+    dmrg,
+    validate_mps_sector,
+)
+
+__all__ = ["AutoMPO", "build_auto_mpo", "spin_half_ops", "spin_one_ops", "DMRGConfig", "DMRGResult", "SweepOps",
+           "build_mpo_heisenberg", "build_random_symmetric_mps", "compute_mps_sector", "dmrg",
+           "validate_mps_sector"]

@@ -164,14 +164,11 @@ def contract_pair(A: SymmetricTensor, B: SymmetricTensor, sa: str, sb: str, so: 
     else:
         A = A.transpose(tuple(sa.index(c) for c in fa + cont))
         sa, a_kfirst = "".join(fa + cont), 0
-    # B layout (contracted order fixed by A)
-    if list(sb) == cont + fb:
-        b_kfirst = 1
-    elif list(sb) == fb + cont:
-        b_kfirst = 0
-    else:
+    # B layout: always [contracted (in A's order)..., free...]
+    b_kfirst = 1
+    if list(sb) != cont + fb:
         B = B.transpose(tuple(sb.index(c) for c in cont + fb))
-        sb, b_kfirst = "".join(cont + fb), 1
+        sb = "".join(cont + fb)
     nat_indices = tuple(ix_of[c] for c in nat)
     plan, keys, shapes, offs, tot = plans.pairwise_plan(A, B, a_kfirst, b_kfirst, len(cont), nat_indices, mode,
                                                         target)

@@ -16,27 +16,31 @@ import numpy as np
 from tenax_amd import _lib
 
 _MAX_PLANS = 1024
+_MAX_FUSED = 48  # fused DMRG plans own intermediate buffers: keep fewer
 _cache: "OrderedDict[tuple, object]" = OrderedDict()
+_fused: "OrderedDict[tuple, object]" = OrderedDict()
 _lock = threading.Lock()
 
 
 def _get(key, build):
+    store, cap = (_fused, _MAX_FUSED) if key[0] in ("env", "matvec") else (_cache, _MAX_PLANS)
     with _lock:
-        hit = _cache.get(key)
+        hit = store.get(key)
         if hit is not None:
-            _cache.move_to_end(key)
+            store.move_to_end(key)
             return hit
     val = build()
     with _lock:
-        _cache[key] = val
-        while len(_cache) > _MAX_PLANS:
-            _cache.popitem(last=False)
+        store[key] = val
+        while len(store) > cap:
+            store.popitem(last=False)
     return val
 
 
 def clear_plan_cache():
     with _lock:
         _cache.clear()
+        _fused.clear()
 
 
 def _struct_sig(t):
@@ -85,5 +89,60 @@ def transpose_plan(T, axes):
         plan = _lib.Plan(ptr)
         keys, shapes, offs, tot = plan.output()
         return plan, keys, shapes, offs, tot
+
+    return _get(key, build)
+
+
+def _table(T):
+    return _lib.BlockTable(T.keys, T.shapes, T.offsets, T.ndim)
+
+
+def env_plan(side: int, E, A, W, out_indices):
+    """Environment-update plan (left: side 0, right: side 1)."""
+    key = ("env", side, _struct_sig(E), _struct_sig(A), _struct_sig(W), _legs_sig(out_indices))
+
+    def build():
+        lib = _lib.load()
+        te, ta, tw = _table(E), _table(A), _table(W)
+        legs = _lib.LegArray([(*ix.sectors(), int(ix.flow)) for ix in out_indices])
+        ptr = _lib.C.c_void_p()
+        _lib.check(lib.tnx_env_plan(side, _lib.C.byref(te.s), _lib.C.byref(ta.s), _lib.C.byref(tw.s), legs.arr,
+                                    _lib.stream_handle(), _lib.C.byref(ptr)))
+        plan = _lib.Plan(ptr)
+        keys, shapes, offs, tot = plan.output()
+        return plan, keys, shapes, offs, tot
+
+    return _get(key, build)
+
+
+def matvec_plan(L, theta_keys, theta_shapes, theta_offsets, Wl, Wr, R):
+    """Fused two-site matvec plan over the full theta sector layout.
+
+    Returns (Plan, reach_src, reach_dst): theta block -> output block edges.
+    """
+    key = ("matvec", _struct_sig(L), theta_keys.tobytes(), theta_shapes.tobytes(), _struct_sig(Wl),
+           _struct_sig(Wr), _struct_sig(R))
+
+    def build():
+        lib = _lib.load()
+        tl, tt, twl, twr, tr = (_table(L), _lib.BlockTable(theta_keys, theta_shapes, theta_offsets, 4), _table(Wl),
+                                _table(Wr), _table(R))
+        ptr = _lib.C.c_void_p()
+        _lib.check(lib.tnx_matvec_plan(_lib.C.byref(tl.s), _lib.C.byref(tt.s), _lib.C.byref(twl.s),
+                                       _lib.C.byref(twr.s), _lib.C.byref(tr.s), _lib.stream_handle(),
+                                       _lib.C.byref(ptr)))
+        plan = _lib.Plan(ptr)
+        ne = _lib.C.c_int32()
+        cap = 1
+        while True:
+            src = np.zeros(cap, np.int32)
+            dst = np.zeros(cap, np.int32)
+            rc = lib.tnx_matvec_reach(plan.ptr, _lib.C.byref(ne), _lib.i32p(src), _lib.i32p(dst), cap)
+            if rc == _lib.TNX_ERR_CAPACITY:
+                cap = ne.value
+                continue
+            _lib.check(rc)
+            break
+        return plan, src[: ne.value].copy(), dst[: ne.value].copy()
 
     return _get(key, build)

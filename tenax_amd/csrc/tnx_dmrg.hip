@@ -1,0 +1,1334 @@
+// DMRG contractions on the device: fused two-site matvec, environment updates
+// and a device-resident Lanczos solver.
+//
+// Reference structure replaced (src/tenax/algorithms/dmrg.py):
+//   _blockwise_contract (1048-1151): depth-first enumeration of 4-/5-block
+//     combinations, one opt_einsum dispatch per combination, per-key list sum;
+//   _update_left/right_env_symmetric (1165-1232), matvec closure (1282-1289);
+//   _lanczos_solve_tensor (978-1045): Python loop with two host syncs per step.
+// Here the combinations are enumerated by GPU kernels at plan time and turned
+// into three launches per application (grouped GEMM, coefficient mix, grouped
+// GEMM), and the whole Lanczos iteration stays on the device.
+#include "tnx_internal.h"
+
+#include <rocprim/rocprim.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <mutex>
+
+#include "../../include/tnx_dmrg.h"
+
+namespace tnx {
+
+void launch_prepared(int a_lay, int b_lay, int compute, int n_tasks, hipStream_t s, const GemmTask *tasks,
+                     const GemmOut *outs, const GemmPair *pairs, const double *A, const double *B, double *C,
+                     bool accumulate);
+
+static unsigned nb256(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+template <class T>
+static void scan_ex(const T *in, T *out, size_t n, hipStream_t s, DevBuf<char> &tmp) {
+  if (n == 0) return;
+  size_t tb = 0;
+  TNX_CHECK_HIP(rocprim::exclusive_scan(nullptr, tb, in, out, T(0), n, rocprim::plus<T>(), s));
+  tmp.alloc(tb ? tb : 1);
+  TNX_CHECK_HIP(rocprim::exclusive_scan(tmp.p, tb, in, out, T(0), n, rocprim::plus<T>(), s));
+}
+
+template <class K, class V>
+static void sort_kv(const K *kin, K *kout, const V *vin, V *vout, size_t n, int end_bit, hipStream_t s,
+                    DevBuf<char> &tmp) {
+  if (n == 0) return;
+  size_t tb = 0;
+  TNX_CHECK_HIP(rocprim::radix_sort_pairs(nullptr, tb, kin, kout, vin, vout, n, 0, end_bit, s));
+  tmp.alloc(tb ? tb : 1);
+  TNX_CHECK_HIP(rocprim::radix_sort_pairs(tmp.p, tb, kin, kout, vin, vout, n, 0, end_bit, s));
+}
+
+static int nbits(uint64_t v) {
+  int b = 1;
+  while (b < 64 && (v >> b) != 0) ++b;
+  return b;
+}
+
+template <class T>
+static T last_of(const DevBuf<T> &b, size_t n, hipStream_t s) {
+  T v{};
+  TNX_CHECK_HIP(hipMemcpyAsync(&v, b.p + n - 1, sizeof(T), hipMemcpyDeviceToHost, s));
+  TNX_CHECK_HIP(hipStreamSynchronize(s));
+  return v;
+}
+
+// ------------------------------------------------------------------ block tables on device
+struct DevTable {
+  int32_t rank = 0, n = 0;
+  DevBuf<int32_t> keys, shapes;
+  DevBuf<int64_t> offs;
+  std::vector<int32_t> hk, hs;
+  std::vector<int64_t> ho;
+  void up(const tnx_blocks *t, int want_rank, const char *name, hipStream_t s) {
+    TNX_REQUIRE(t, TNX_ERR_ARG, std::string("null block table ") + name);
+    TNX_REQUIRE(t->rank == want_rank, TNX_ERR_BAD_STRUCTURE, std::string("wrong rank for ") + name);
+    rank = t->rank;
+    n = t->n_blocks;
+    hk.assign(t->keys, t->keys + (size_t)n * rank);
+    hs.assign(t->shapes, t->shapes + (size_t)n * rank);
+    ho.assign(t->offsets, t->offsets + n);
+    keys.upload(hk.data(), hk.size(), s);
+    shapes.upload(hs.data(), hs.size(), s);
+    offs.upload(ho.data(), ho.size(), s);
+  }
+};
+
+struct TView {
+  const int32_t *k;
+  const int32_t *s;
+  const int64_t *o;
+  int32_t n, r;
+  __device__ __forceinline__ int32_t key(int i, int l) const { return k[(int64_t)i * r + l]; }
+  __device__ __forceinline__ int32_t dim(int i, int l) const { return s[(int64_t)i * r + l]; }
+};
+
+static TView view(const DevTable &t) { return TView{t.keys.p, t.shapes.p, t.offs.p, t.n, t.rank}; }
+
+__device__ __forceinline__ int find_key4(const TView &V, int32_t a, int32_t b, int32_t c, int32_t d) {
+  int lo = 0, hi = V.n - 1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int32_t *k = V.k + (int64_t)mid * 4;
+    int cmp = (k[0] != a) ? (k[0] < a ? -1 : 1)
+              : (k[1] != b) ? (k[1] < b ? -1 : 1)
+              : (k[2] != c) ? (k[2] < c ? -1 : 1)
+              : (k[3] != d) ? (k[3] < d ? -1 : 1) : 0;
+    if (cmp == 0) return mid;
+    if (cmp < 0) lo = mid + 1; else hi = mid - 1;
+  }
+  return -1;
+}
+
+__device__ __forceinline__ int find_key3(const TView &V, int32_t a, int32_t b, int32_t c) {
+  int lo = 0, hi = V.n - 1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int32_t *k = V.k + (int64_t)mid * 3;
+    int cmp = (k[0] != a) ? (k[0] < a ? -1 : 1) : (k[1] != b) ? (k[1] < b ? -1 : 1) : (k[2] != c) ? (k[2] < c ? -1 : 1) : 0;
+    if (cmp == 0) return mid;
+    if (cmp < 0) lo = mid + 1; else hi = mid - 1;
+  }
+  return -1;
+}
+
+// ------------------------------------------------------------------ terms
+// One block combination after the first contraction: X item (pair index of
+// the stage-1 product), output block, stage-2 partner block, MPO blocks.
+struct TermRec {
+  int32_t pair;  // stage-1 candidate pair index (i * n2 + j)
+  int32_t out;   // output block (or candidate-key index for env plans)
+  int32_t partner;
+  int32_t w1, w2;
+};
+
+// kind 0 = matvec, 1 = environment (left form; right environments are
+// evaluated as mirrored left environments by the caller)
+struct EnumSpec {
+  int kind;
+  TView P1, P2;   // stage-1 operands: matvec (L, theta); left (E, A); right (E, A)
+  TView W1, W2;   // MPO blocks (W2 only for matvec)
+  TView Part;     // stage-2 partner: matvec R; env A (bra)
+  TView Out;      // output key table (matvec: theta layout; env: candidate keys)
+};
+
+// Enumerate the combinations of one stage-1 pair; emit() is called per term.
+template <class F>
+__device__ void enumerate_pair(const EnumSpec &S, int i, int j, F &&emit) {
+  if (S.kind == 0) {
+    // L[a,b,c] th[a,p,q,d] Wl[b,p,s,e] Wr[e,q,t,f] R[d,f,g] -> out[c,s,t,g]
+    if (S.P1.key(i, 0) != S.P2.key(j, 0)) return;
+    const int32_t qb = S.P1.key(i, 1), qc = S.P1.key(i, 2);
+    const int32_t qp = S.P2.key(j, 1), qq = S.P2.key(j, 2), qd = S.P2.key(j, 3);
+    for (int w = 0; w < S.W1.n; ++w) {
+      if (S.W1.key(w, 0) != qb || S.W1.key(w, 1) != qp) continue;
+      const int32_t qs = S.W1.key(w, 2), qe = S.W1.key(w, 3);
+      for (int v = 0; v < S.W2.n; ++v) {
+        if (S.W2.key(v, 0) != qe || S.W2.key(v, 1) != qq) continue;
+        const int32_t qt = S.W2.key(v, 2), qf = S.W2.key(v, 3);
+        for (int r = 0; r < S.Part.n; ++r) {
+          if (S.Part.key(r, 0) != qd || S.Part.key(r, 1) != qf) continue;
+          const int32_t qg = S.Part.key(r, 2);
+          const int o = find_key4(S.Out, qc, qs, qt, qg);
+          emit(o, r, w, v);
+        }
+      }
+    }
+  } else if (S.kind == 1) {
+    // E[a,b,c] A[a,p,d] W[b,p,x,e] A[c,x,f] -> new[d,e,f]
+    if (S.P1.key(i, 0) != S.P2.key(j, 0)) return;
+    const int32_t qb = S.P1.key(i, 1), qc = S.P1.key(i, 2);
+    const int32_t qp = S.P2.key(j, 1), qd = S.P2.key(j, 2);
+    for (int w = 0; w < S.W1.n; ++w) {
+      if (S.W1.key(w, 0) != qb || S.W1.key(w, 1) != qp) continue;
+      const int32_t qx = S.W1.key(w, 2), qe = S.W1.key(w, 3);
+      for (int r = 0; r < S.Part.n; ++r) {
+        if (S.Part.key(r, 0) != qc || S.Part.key(r, 1) != qx) continue;
+        const int32_t qf = S.Part.key(r, 2);
+        const int o = find_key3(S.Out, qd, qe, qf);
+        emit(o, r, w, -1);
+      }
+    }
+  }
+}
+
+__global__ void count_terms_kernel(EnumSpec S, int64_t *__restrict__ cnt, int32_t *__restrict__ bad) {
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t np = (int64_t)S.P1.n * S.P2.n;
+  if (idx >= np) return;
+  const int i = (int)(idx / S.P2.n), j = (int)(idx % S.P2.n);
+  int64_t c = 0;
+  enumerate_pair(S, i, j, [&](int o, int, int, int) {
+    if (o < 0) *bad = 1; else ++c;
+  });
+  cnt[idx] = c;
+}
+
+__global__ void write_terms_kernel(EnumSpec S, const int64_t *__restrict__ toff, TermRec *__restrict__ terms) {
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t np = (int64_t)S.P1.n * S.P2.n;
+  if (idx >= np) return;
+  const int i = (int)(idx / S.P2.n), j = (int)(idx % S.P2.n);
+  int64_t t = toff[idx];
+  enumerate_pair(S, i, j, [&](int o, int r, int w, int v) {
+    if (o >= 0) terms[t++] = TermRec{(int32_t)idx, o, r, w, v};
+  });
+}
+
+// ------------------------------------------------------------------ mix
+struct MixTerm {
+  int64_t x_off;
+  int64_t sb, su, sv;
+  int64_t coef_off;
+  int32_t nb, pad;
+};
+
+struct MixItem {
+  int64_t y_off;
+  int32_t D0, D1, D2, ca;
+  int32_t term_begin, term_end;
+};
+
+struct CoefSpec {
+  int64_t w1_off, w2_off, coef_off;
+  int32_t kind, nb, nmid, nj;
+};
+
+__global__ void coef_kernel(int64_t n, const CoefSpec *__restrict__ cs, const double *__restrict__ W1,
+                            const double *__restrict__ W2, double *__restrict__ coef) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const CoefSpec c = cs[t];
+  for (int b = 0; b < c.nb; ++b)
+    for (int j = 0; j < c.nj; ++j) {
+      double v;
+      if (c.kind == 0) {  // sum_e Wl[b,e] Wr[e,f]
+        v = 0.0;
+        for (int e = 0; e < c.nmid; ++e) v += W1[c.w1_off + (int64_t)b * c.nmid + e] * W2[c.w2_off + (int64_t)e * c.nj + j];
+      } else {  // W[b,0,0,e]
+        v = W1[c.w1_off + (int64_t)b * c.nj + j];
+      }
+      coef[c.coef_off + (int64_t)b * c.nj + j] = v;
+    }
+}
+
+__global__ void mix_kernel(const MixItem *__restrict__ items, const int32_t *__restrict__ task_item,
+                           const int64_t *__restrict__ task_start, const MixTerm *__restrict__ terms,
+                           const double *__restrict__ coef, const double *__restrict__ X, double *__restrict__ Y) {
+  const MixItem it = items[task_item[blockIdx.x]];
+  const int64_t e = task_start[blockIdx.x] + threadIdx.x;
+  const int64_t size = (int64_t)it.D0 * it.D1 * it.D2;
+  if (e >= size) return;
+  const int i2 = (int)(e % it.D2);
+  const int64_t r = e / it.D2;
+  const int i1 = (int)(r % it.D1);
+  const int i0 = (int)(r / it.D1);
+  int j, u, v;
+  if (it.ca == 0) { j = i0; u = i1; v = i2; }
+  else if (it.ca == 1) { j = i1; u = i0; v = i2; }
+  else { j = i2; u = i0; v = i1; }
+  const int nj = it.ca == 0 ? it.D0 : (it.ca == 1 ? it.D1 : it.D2);
+  double acc = 0.0;
+  for (int t = it.term_begin; t < it.term_end; ++t) {
+    const MixTerm m = terms[t];
+    const double *xp = X + m.x_off + (int64_t)u * m.su + (int64_t)v * m.sv;
+    const double *cp = coef + m.coef_off + j;
+    for (int b = 0; b < m.nb; ++b) acc += cp[(int64_t)b * nj] * xp[(int64_t)b * m.sb];
+  }
+  Y[it.y_off + e] = acc;
+}
+
+// ------------------------------------------------------------------ plan-building kernels
+// Stage-1 program entries for used pairs.
+struct Stage1Spec {
+  int kind;
+  TView P1, P2;
+};
+
+__device__ __forceinline__ void stage1_dims(const Stage1Spec &S, int i, int j, int64_t &M, int64_t &N, int64_t &K,
+                                            int64_t &a_off, int64_t &b_off, int32_t &lda, int32_t &ldb) {
+  // A-op: L/E block [a,(b,c)] (KM); B-op: theta/site block [a,(p..,d)] (KN)
+  K = S.P1.dim(i, 0);
+  M = (int64_t)S.P1.dim(i, 1) * S.P1.dim(i, 2);
+  N = 1;
+  for (int l = 1; l < S.P2.r; ++l) N *= S.P2.dim(j, l);
+  a_off = S.P1.o[i];
+  b_off = S.P2.o[j];
+  lda = (int32_t)M;
+  ldb = (int32_t)N;
+}
+
+__global__ void used_pairs_kernel(int64_t np, const int64_t *__restrict__ cnt, int64_t *__restrict__ used) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p < np) used[p] = cnt[p] > 0 ? 1 : 0;
+}
+
+__global__ void x_sizes_kernel(int64_t np, Stage1Spec S, const int64_t *__restrict__ used, int64_t *__restrict__ xs) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  if (!used[p]) { xs[p] = 0; return; }
+  int64_t M, N, K, ao, bo;
+  int32_t lda, ldb;
+  stage1_dims(S, (int)(p / S.P2.n), (int)(p % S.P2.n), M, N, K, ao, bo, lda, ldb);
+  xs[p] = M * N;
+}
+
+__global__ void build_stage1_kernel(int64_t np, Stage1Spec S, const int64_t *__restrict__ used,
+                                    const int64_t *__restrict__ xrank, const int64_t *__restrict__ xoff,
+                                    GemmOut *__restrict__ outs, GemmPair *__restrict__ pairs) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= np || !used[p]) return;
+  int64_t M, N, K, ao, bo;
+  int32_t lda, ldb;
+  stage1_dims(S, (int)(p / S.P2.n), (int)(p % S.P2.n), M, N, K, ao, bo, lda, ldb);
+  const int64_t x = xrank[p];
+  outs[x] = GemmOut{xoff[p], (int32_t)M, (int32_t)N, (int32_t)N, (int32_t)x, (int32_t)x + 1, 0};
+  pairs[x] = GemmPair{ao, bo, (int32_t)K, lda, ldb, 0};
+}
+
+__global__ void term_codes_kernel(int64_t nt, const TermRec *__restrict__ terms, int32_t npart,
+                                  uint64_t *__restrict__ code, uint32_t *__restrict__ ids) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  code[t] = (uint64_t)terms[t].out * (uint64_t)npart + (uint64_t)terms[t].partner;
+  ids[t] = (uint32_t)t;
+}
+
+__global__ void new_item_kernel(int64_t nt, const uint64_t *__restrict__ scode, int64_t *__restrict__ flag) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < nt) flag[k] = (k == 0 || scode[k] != scode[k - 1]) ? 1 : 0;
+}
+
+struct ItemSpec {
+  int kind;
+  TView P1, P2, W1, W2, Part, Out;
+};
+
+// Per mix item: dims (D0, D1, D2), coefficient axis; the stage-2 pair.
+__device__ __forceinline__ void item_dims(const ItemSpec &S, int o, int r, int &D0, int &D1, int &D2, int &ca) {
+  if (S.kind == 0) {
+    // Y[c, d, f]: c from out (theta layout leg 0), (d, f) from R block
+    D0 = S.Out.dim(o, 0);
+    D1 = S.Part.dim(r, 0);
+    D2 = S.Part.dim(r, 1);
+    ca = 2;
+  } else {
+    // Z[d, e, c]: d, e from out key dims, c from bra block A[c,x,f]
+    D0 = S.Out.dim(o, 0);
+    D1 = S.Out.dim(o, 1);
+    D2 = S.Part.dim(r, 0);
+    ca = 1;
+  }
+}
+
+__global__ void item_sizes_kernel(int64_t nt, ItemSpec S, const TermRec *__restrict__ terms,
+                                  const uint32_t *__restrict__ sids, const int64_t *__restrict__ nflag,
+                                  int64_t *__restrict__ isize) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= nt) return;
+  if (!nflag[k]) { isize[k] = 0; return; }
+  const TermRec tr = terms[sids[k]];
+  int D0, D1, D2, ca;
+  item_dims(S, tr.out, tr.partner, D0, D1, D2, ca);
+  isize[k] = (int64_t)D0 * D1 * D2;
+}
+
+__global__ void build_items_kernel(int64_t nt, ItemSpec S, const TermRec *__restrict__ terms,
+                                   const uint32_t *__restrict__ sids, const uint64_t *__restrict__ scode,
+                                   const int64_t *__restrict__ nflag, const int64_t *__restrict__ irank,
+                                   const int64_t *__restrict__ ioff, const int64_t *__restrict__ xrank,
+                                   const int64_t *__restrict__ xoff_by_pair, const int64_t *__restrict__ coef_off,
+                                   MixItem *__restrict__ items, MixTerm *__restrict__ mterms,
+                                   CoefSpec *__restrict__ cspec, int32_t *__restrict__ item_out,
+                                   int32_t *__restrict__ item_part) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= nt) return;
+  const TermRec tr = terms[sids[k]];
+  const int64_t it = irank[k] + nflag[k] - 1;  // item of sorted term k
+  int D0, D1, D2, ca;
+  item_dims(S, tr.out, tr.partner, D0, D1, D2, ca);
+  if (nflag[k]) {
+    int64_t e = k + 1;
+    while (e < nt && scode[e] == scode[k]) ++e;
+    items[it] = MixItem{ioff[k], D0, D1, D2, ca, (int32_t)k, (int32_t)e};
+    item_out[it] = tr.out;
+    item_part[it] = tr.partner;
+  }
+  const int i = tr.pair / S.P2.n, j = tr.pair % S.P2.n;
+  MixTerm m{};
+  m.x_off = xoff_by_pair[tr.pair];
+  m.coef_off = coef_off[k];
+  CoefSpec c{};
+  c.coef_off = coef_off[k];
+  c.kind = S.kind;
+  if (S.kind == 0) {
+    const int nb = S.P1.dim(i, 1), mc = S.P1.dim(i, 2), md = S.P2.dim(j, 3);
+    m.nb = nb;
+    m.sb = (int64_t)mc * md;
+    m.su = md;  // c
+    m.sv = 1;   // d
+    c.w1_off = S.W1.o[tr.w1];
+    c.w2_off = S.W2.o[tr.w2];
+    c.nb = nb;
+    c.nmid = S.W1.dim(tr.w1, 3);
+    c.nj = S.W2.dim(tr.w2, 3);
+  } else {
+    const int nb = S.P1.dim(i, 1), mc = S.P1.dim(i, 2), md = S.P2.dim(j, 2);
+    m.nb = nb;
+    m.sb = (int64_t)mc * md;
+    m.su = 1;   // d
+    m.sv = md;  // c
+    c.w1_off = S.W1.o[tr.w1];
+    c.nb = nb;
+    c.nj = S.W1.dim(tr.w1, 3);
+  }
+  (void)xrank;
+  mterms[k] = m;
+  cspec[k] = c;
+}
+
+__global__ void coef_sizes_kernel(int64_t nt, ItemSpec S, const TermRec *__restrict__ terms,
+                                  const uint32_t *__restrict__ sids, int64_t *__restrict__ csz) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= nt) return;
+  const TermRec tr = terms[sids[k]];
+  const int i = tr.pair / S.P2.n;
+  const int nb = S.P1.dim(i, 1);
+  int nj;
+  if (S.kind == 0) nj = S.W2.dim(tr.w2, 3);
+  else nj = S.W1.dim(tr.w1, 3);
+  csz[k] = (int64_t)nb * nj;
+}
+
+// Stage-2 pairs (one per mix item), grouped by output block.
+__global__ void build_stage2_kernel(int64_t ni, ItemSpec S, const MixItem *__restrict__ items,
+                                    const int32_t *__restrict__ item_part, const int32_t *__restrict__ item_oid,
+                                    GemmPair *__restrict__ pairs, int32_t *__restrict__ pbeg,
+                                    int32_t *__restrict__ pend) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= ni) return;
+  const MixItem it = items[k];
+  const int r = item_part[k];
+  GemmPair g{};
+  g.a_off = it.y_off;
+  g.b_off = S.Part.o[r];
+  if (S.kind == 0) {
+    g.K = it.D1 * it.D2;  // (d, f)
+    g.lda = g.K;
+    g.ldb = S.Part.dim(r, 2);  // g
+  } else {
+    g.K = it.D2;  // c (x dim 1)
+    g.lda = g.K;
+    g.ldb = S.Part.dim(r, 2);  // f
+  }
+  pairs[k] = g;
+  const int32_t o = item_oid[k];
+  if (k == 0 || item_oid[k - 1] != o) pbeg[o] = (int32_t)k;
+  if (k == ni - 1 || item_oid[k + 1] != o) pend[o] = (int32_t)(k + 1);
+}
+
+__global__ void map_out_ids_kernel(int64_t ni, const int32_t *__restrict__ item_out, const int64_t *__restrict__ out_id,
+                                   int32_t *__restrict__ item_oid) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < ni) item_oid[k] = out_id ? (int32_t)out_id[item_out[k]] : item_out[k];
+}
+
+__global__ void mark_present_kernel(int64_t nt, const TermRec *__restrict__ terms, int32_t *__restrict__ present) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t < nt) present[terms[t].out] = 1;
+}
+
+__global__ void i32_to_i64_kernel(int64_t n, const int32_t *__restrict__ a, int64_t *__restrict__ b) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+__global__ void env_out_tables_kernel(int64_t nV, const int32_t *__restrict__ present, const int64_t *__restrict__ out_id,
+                                      const int32_t *__restrict__ V, const int32_t *__restrict__ Vs,
+                                      int32_t *__restrict__ okeys, int32_t *__restrict__ oshapes,
+                                      int64_t *__restrict__ osize) {
+  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v >= nV || !present[v]) return;
+  const int64_t o = out_id[v];
+  int64_t sz = 1;
+  for (int l = 0; l < 3; ++l) {
+    okeys[o * 3 + l] = V[v * 3 + l];
+    oshapes[o * 3 + l] = Vs[v * 3 + l];
+    sz *= Vs[v * 3 + l];
+  }
+  osize[o] = sz;
+}
+
+__global__ void stage2_outs_kernel(int64_t nout, int kind, TView OutT, const int64_t *__restrict__ ooff,
+                                   const int32_t *__restrict__ oshapes, const int32_t *__restrict__ pbeg,
+                                   const int32_t *__restrict__ pend, GemmOut *__restrict__ outs) {
+  const int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (o >= nout) return;
+  GemmOut g{};
+  if (kind == 0) {
+    g.c_off = OutT.o[o];
+    g.M = OutT.dim(o, 0);  // c (s, t of multiplicity one)
+    g.N = OutT.dim(o, 3);  // g
+  } else {
+    g.c_off = ooff[o];
+    g.M = oshapes[o * 3 + 0] * oshapes[o * 3 + 1];  // (d, e)
+    g.N = oshapes[o * 3 + 2];                       // f
+  }
+  g.ldc = g.N;
+  g.pair_begin = pbeg[o];
+  g.pair_end = pend[o];
+  outs[o] = g;
+}
+
+__global__ void mix_task_count_kernel(int64_t ni, const MixItem *__restrict__ items, int64_t *__restrict__ cnt) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= ni) return;
+  const int64_t sz = (int64_t)items[k].D0 * items[k].D1 * items[k].D2;
+  cnt[k] = (sz + 255) / 256;
+}
+
+__global__ void mix_task_write_kernel(int64_t ni, const int64_t *__restrict__ cnt, const int64_t *__restrict__ off,
+                                      int32_t *__restrict__ titem, int64_t *__restrict__ tstart) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= ni) return;
+  for (int64_t c = 0; c < cnt[k]; ++c) {
+    titem[off[k] + c] = (int32_t)k;
+    tstart[off[k] + c] = c * 256;
+  }
+}
+
+__global__ void zero_kernel(int64_t n, double *__restrict__ p) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+// ------------------------------------------------------------------ the plan
+struct FusedPlan : public tnx_plan_s {
+  int fkind = 0;  // 0 matvec, 1 environment
+  GemmProgram s1, s2;
+  DevBuf<MixItem> items;
+  DevBuf<MixTerm> mterms;
+  DevBuf<CoefSpec> cspec;
+  DevBuf<int32_t> mix_titem;
+  DevBuf<int64_t> mix_tstart;
+  int32_t n_mix_tasks = 0;
+  int64_t n_terms = 0, n_items = 0;
+  DevBuf<double> X, Y, coef;
+  int64_t x_total = 0, y_total = 0, coef_total = 0;
+  int64_t in_total = 0;  // matvec: theta layout size
+  double mix_flops = 0.0;
+  std::vector<int32_t> reach_src, reach_dst;  // matvec: theta block -> out block
+  // per-block path
+  bool pb_ready = false;
+  std::vector<int32_t> pb1_off, pb1_cnt;     // stage-1 tasks per X item (from s1 task list)
+  std::vector<int32_t> pbm_off, pbm_cnt;     // mix tasks per item
+  DevBuf<GemmOut> pb2_outs;
+  DevBuf<GemmTask> pb2_tasks;
+  std::vector<int32_t> pb2_off, pb2_cnt;
+  std::vector<uint8_t> pb2_first;
+  std::vector<GemmOut> pb2_zero;  // outs without pairs: zero-filled
+};
+
+static void build_fused(FusedPlan &P, int kind, DevTable &P1, DevTable &P2, DevTable &W1, DevTable *W2,
+                        DevTable &Part, DevTable *OutT, const tnx_leg *out_legs, hipStream_t s) {
+  P.fkind = kind;
+  P.kind = kind == 0 ? PlanKind::Matvec : PlanKind::EnvLeft;
+  DevBuf<char> tmp;
+  KeySet V;
+  TView outv{};
+  if (kind == 0) {
+    outv = view(*OutT);
+  } else {
+    enumerate_keys(out_legs, 3, 0, true, false, 0, s, V);
+    TNX_REQUIRE(V.n < (int64_t(1) << 31), TNX_ERR_CAPACITY, "too many candidate environment keys");
+    outv = TView{V.keys.p, V.shapes.p, nullptr, (int32_t)V.n, 3};
+  }
+  EnumSpec ES{kind, view(P1), view(P2), view(W1), W2 ? view(*W2) : TView{nullptr, nullptr, nullptr, 0, 4},
+              view(Part), outv};
+  const int64_t np = (int64_t)P1.n * P2.n;
+  TNX_REQUIRE(np < (int64_t(1) << 31), TNX_ERR_CAPACITY, "too many stage-1 block pairs");
+  int64_t nt = 0;
+  DevBuf<int64_t> cnt(std::max<int64_t>(np, 1)), toff(std::max<int64_t>(np, 1));
+  DevBuf<TermRec> terms;
+  if (np > 0) {
+    DevBuf<int32_t> bad(1);
+    TNX_CHECK_HIP(hipMemsetAsync(bad.p, 0, sizeof(int32_t), s));
+    hipLaunchKernelGGL(count_terms_kernel, dim3(nb256(np)), dim3(256), 0, s, ES, cnt.p, bad.p);
+    TNX_CHECK_HIP(hipGetLastError());
+    scan_ex(cnt.p, toff.p, np, s, tmp);
+    nt = last_of(toff, np, s) + last_of(cnt, np, s);
+    int32_t hbad = 0;
+    TNX_CHECK_HIP(hipMemcpy(&hbad, bad.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+    TNX_REQUIRE(!hbad, TNX_ERR_BAD_STRUCTURE, "a block combination reaches a key outside the output layout");
+    TNX_REQUIRE(nt < (int64_t(1) << 31), TNX_ERR_CAPACITY, "too many block combinations");
+    if (nt) {
+      terms.alloc(nt);
+      hipLaunchKernelGGL(write_terms_kernel, dim3(nb256(np)), dim3(256), 0, s, ES, toff.p, terms.p);
+      TNX_CHECK_HIP(hipGetLastError());
+    }
+  }
+  P.n_terms = nt;
+  // ---- outputs
+  int64_t nout = 0;
+  DevBuf<int64_t> out_id;
+  DevBuf<int32_t> okeys, oshapes;
+  DevBuf<int64_t> ooff;
+  if (kind == 0) {
+    nout = OutT->n;
+  } else if (nt > 0) {
+    DevBuf<int32_t> present(V.n);
+    DevBuf<int64_t> pres64(V.n);
+    out_id.alloc(V.n);
+    TNX_CHECK_HIP(hipMemsetAsync(present.p, 0, V.n * sizeof(int32_t), s));
+    hipLaunchKernelGGL(mark_present_kernel, dim3(nb256(nt)), dim3(256), 0, s, nt, terms.p, present.p);
+    hipLaunchKernelGGL(i32_to_i64_kernel, dim3(nb256(V.n)), dim3(256), 0, s, V.n, present.p, pres64.p);
+    scan_ex(pres64.p, out_id.p, V.n, s, tmp);
+    nout = last_of(out_id, V.n, s) + last_of(pres64, V.n, s);
+    okeys.alloc(nout * 3);
+    oshapes.alloc(nout * 3);
+    DevBuf<int64_t> osize(nout);
+    ooff.alloc(nout);
+    hipLaunchKernelGGL(env_out_tables_kernel, dim3(nb256(V.n)), dim3(256), 0, s, V.n, present.p, out_id.p, V.keys.p,
+                       V.shapes.p, okeys.p, oshapes.p, osize.p);
+    scan_ex(osize.p, ooff.p, nout, s, tmp);
+    P.out_total = last_of(ooff, nout, s) + last_of(osize, nout, s);
+    P.out_keys.resize(nout * 3);
+    P.out_shapes.resize(nout * 3);
+    P.out_offsets.resize(nout);
+    okeys.download(P.out_keys.data(), nout * 3, s);
+    oshapes.download(P.out_shapes.data(), nout * 3, s);
+    ooff.download(P.out_offsets.data(), nout, s);
+    TNX_CHECK_HIP(hipStreamSynchronize(s));
+  }
+  P.out_rank = kind == 0 ? 4 : 3;
+  P.out_nblocks = (int32_t)nout;
+  if (kind == 0) {
+    int64_t tot = 0;
+    for (int o = 0; o < OutT->n; ++o) {
+      int64_t sz = 1;
+      for (int l = 0; l < 4; ++l) sz *= OutT->hs[o * 4 + l];
+      tot = std::max(tot, OutT->ho[o] + sz);
+    }
+    P.out_total = tot;
+    P.in_total = tot;
+    P.out_keys = OutT->hk;
+    P.out_shapes = OutT->hs;
+    P.out_offsets = OutT->ho;
+  }
+  // ---- stage 1 (X items = used stage-1 pairs)
+  Stage1Spec S1{kind, view(P1), view(P2)};
+  P.s1.a_lay = LAY_KM;
+  P.s1.b_lay = LAY_KN;
+  DevBuf<int64_t> used(std::max<int64_t>(np, 1)), xrank(std::max<int64_t>(np, 1)), xs(std::max<int64_t>(np, 1)),
+      xoff(std::max<int64_t>(np, 1));
+  int64_t nX = 0;
+  if (nt > 0) {
+    hipLaunchKernelGGL(used_pairs_kernel, dim3(nb256(np)), dim3(256), 0, s, np, cnt.p, used.p);
+    scan_ex(used.p, xrank.p, np, s, tmp);
+    nX = last_of(xrank, np, s) + last_of(used, np, s);
+    hipLaunchKernelGGL(x_sizes_kernel, dim3(nb256(np)), dim3(256), 0, s, np, S1, used.p, xs.p);
+    scan_ex(xs.p, xoff.p, np, s, tmp);
+    P.x_total = last_of(xoff, np, s) + last_of(xs, np, s);
+    P.s1.outs.alloc(nX);
+    P.s1.pairs.alloc(nX);
+    hipLaunchKernelGGL(build_stage1_kernel, dim3(nb256(np)), dim3(256), 0, s, np, S1, used.p, xrank.p, xoff.p,
+                       P.s1.outs.p, P.s1.pairs.p);
+    TNX_CHECK_HIP(hipGetLastError());
+  }
+  P.s1.n_outs = (int32_t)nX;
+  P.s1.n_pairs = (int32_t)nX;
+  finalize_program(P.s1, s);
+  // ---- mix items: terms sorted by (out, partner)
+  ItemSpec IS{kind, view(P1), view(P2), view(W1), W2 ? view(*W2) : TView{nullptr, nullptr, nullptr, 0, 4},
+              view(Part), outv};
+  const int32_t npart = Part.n;
+  int64_t nI = 0;
+  DevBuf<int32_t> item_out, item_part, item_oid;
+  if (nt > 0) {
+    DevBuf<uint64_t> code(nt), scode(nt);
+    DevBuf<uint32_t> ids(nt), sids(nt);
+    hipLaunchKernelGGL(term_codes_kernel, dim3(nb256(nt)), dim3(256), 0, s, nt, terms.p, npart, code.p, ids.p);
+    const uint64_t maxcode = (uint64_t)std::max<int64_t>(kind == 0 ? OutT->n : V.n, 1) * (uint64_t)std::max(npart, 1);
+    sort_kv(code.p, scode.p, ids.p, sids.p, nt, nbits(maxcode), s, tmp);
+    DevBuf<int64_t> nflag(nt), irank(nt), isize(nt), ioff(nt), csz(nt), coff(nt);
+    hipLaunchKernelGGL(new_item_kernel, dim3(nb256(nt)), dim3(256), 0, s, nt, scode.p, nflag.p);
+    scan_ex(nflag.p, irank.p, nt, s, tmp);
+    nI = last_of(irank, nt, s) + last_of(nflag, nt, s);
+    hipLaunchKernelGGL(item_sizes_kernel, dim3(nb256(nt)), dim3(256), 0, s, nt, IS, terms.p, sids.p, nflag.p,
+                       isize.p);
+    scan_ex(isize.p, ioff.p, nt, s, tmp);
+    P.y_total = last_of(ioff, nt, s) + last_of(isize, nt, s);
+    hipLaunchKernelGGL(coef_sizes_kernel, dim3(nb256(nt)), dim3(256), 0, s, nt, IS, terms.p, sids.p, csz.p);
+    scan_ex(csz.p, coff.p, nt, s, tmp);
+    P.coef_total = last_of(coff, nt, s) + last_of(csz, nt, s);
+    P.items.alloc(nI);
+    P.mterms.alloc(nt);
+    P.cspec.alloc(nt);
+    item_out.alloc(nI);
+    item_part.alloc(nI);
+    item_oid.alloc(nI);
+    hipLaunchKernelGGL(build_items_kernel, dim3(nb256(nt)), dim3(256), 0, s, nt, IS, terms.p, sids.p, scode.p,
+                       nflag.p, irank.p, ioff.p, xrank.p, xoff.p, coff.p, P.items.p, P.mterms.p, P.cspec.p,
+                       item_out.p, item_part.p);
+    TNX_CHECK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(map_out_ids_kernel, dim3(nb256(nI)), dim3(256), 0, s, nI, item_out.p,
+                       kind == 0 ? (const int64_t *)nullptr : (const int64_t *)out_id.p, item_oid.p);
+    // reach relation / mix flops (host, from the term list)
+    std::vector<TermRec> ht(nt);
+    TNX_CHECK_HIP(hipMemcpyAsync(ht.data(), terms.p, nt * sizeof(TermRec), hipMemcpyDeviceToHost, s));
+    std::vector<MixTerm> hm(nt);
+    TNX_CHECK_HIP(hipMemcpyAsync(hm.data(), P.mterms.p, nt * sizeof(MixTerm), hipMemcpyDeviceToHost, s));
+    std::vector<MixItem> hi(nI);
+    TNX_CHECK_HIP(hipMemcpyAsync(hi.data(), P.items.p, nI * sizeof(MixItem), hipMemcpyDeviceToHost, s));
+    TNX_CHECK_HIP(hipStreamSynchronize(s));
+    double mf = 0;
+    for (const auto &it : hi)
+      for (int k = it.term_begin; k < it.term_end; ++k) mf += 2.0 * hm[k].nb * (double)it.D0 * it.D1 * it.D2;
+    P.mix_flops = mf;
+    if (kind == 0) {
+      std::vector<std::pair<int32_t, int32_t>> rel;
+      rel.reserve(nt);
+      for (const auto &t : ht) rel.emplace_back(t.pair % P2.n, t.out);
+      std::sort(rel.begin(), rel.end());
+      rel.erase(std::unique(rel.begin(), rel.end()), rel.end());
+      P.reach_src.clear();
+      P.reach_dst.clear();
+      for (auto &e : rel) {
+        P.reach_src.push_back(e.first);
+        P.reach_dst.push_back(e.second);
+      }
+    }
+  }
+  P.n_items = nI;
+  // ---- stage 2
+  P.s2.a_lay = LAY_MK;
+  P.s2.b_lay = LAY_KN;
+  if (nout > 0) {
+    DevBuf<int32_t> pbeg(nout), pend(nout);
+    TNX_CHECK_HIP(hipMemsetAsync(pbeg.p, 0, nout * sizeof(int32_t), s));
+    TNX_CHECK_HIP(hipMemsetAsync(pend.p, 0, nout * sizeof(int32_t), s));
+    P.s2.pairs.alloc(std::max<int64_t>(nI, 1));
+    if (nI > 0)
+      hipLaunchKernelGGL(build_stage2_kernel, dim3(nb256(nI)), dim3(256), 0, s, nI, IS, P.items.p, item_part.p,
+                         item_oid.p, P.s2.pairs.p, pbeg.p, pend.p);
+    P.s2.outs.alloc(nout);
+    hipLaunchKernelGGL(stage2_outs_kernel, dim3(nb256(nout)), dim3(256), 0, s, nout, kind,
+                       kind == 0 ? view(*OutT) : TView{}, ooff.p, oshapes.p, pbeg.p, pend.p, P.s2.outs.p);
+    TNX_CHECK_HIP(hipGetLastError());
+  }
+  P.s2.n_outs = (int32_t)nout;
+  P.s2.n_pairs = (int32_t)nI;
+  finalize_program(P.s2, s);
+  // ---- mix tasks
+  if (nI > 0) {
+    DevBuf<int64_t> mc(nI), mo(nI);
+    hipLaunchKernelGGL(mix_task_count_kernel, dim3(nb256(nI)), dim3(256), 0, s, nI, P.items.p, mc.p);
+    scan_ex(mc.p, mo.p, nI, s, tmp);
+    const int64_t ntask = last_of(mo, nI, s) + last_of(mc, nI, s);
+    TNX_REQUIRE(ntask < (int64_t(1) << 31), TNX_ERR_CAPACITY, "too many mix tasks");
+    P.mix_titem.alloc(ntask);
+    P.mix_tstart.alloc(ntask);
+    hipLaunchKernelGGL(mix_task_write_kernel, dim3(nb256(nI)), dim3(256), 0, s, nI, mc.p, mo.p, P.mix_titem.p,
+                       P.mix_tstart.p);
+    P.n_mix_tasks = (int32_t)ntask;
+    // host copies for the per-block path
+    std::vector<int64_t> hmc(nI), hmo(nI);
+    mc.download(hmc.data(), nI, s);
+    mo.download(hmo.data(), nI, s);
+    TNX_CHECK_HIP(hipStreamSynchronize(s));
+    P.pbm_off.assign(nI, 0);
+    P.pbm_cnt.assign(nI, 0);
+    for (int64_t k = 0; k < nI; ++k) {
+      P.pbm_off[k] = (int32_t)hmo[k];
+      P.pbm_cnt[k] = (int32_t)hmc[k];
+    }
+  }
+  P.X.alloc(std::max<int64_t>(P.x_total, 1));
+  P.Y.alloc(std::max<int64_t>(P.y_total, 1));
+  P.coef.alloc(std::max<int64_t>(P.coef_total, 1));
+  TNX_CHECK_HIP(hipStreamSynchronize(s));
+  P.flops = P.s1.flops + P.s2.flops + P.mix_flops;
+  int64_t in_bytes = 0;
+  for (auto *t : {&P1, &P2, &W1, &Part}) {
+    int64_t m = 0;
+    for (int i = 0; i < t->n; ++i) {
+      int64_t sz = 1;
+      for (int l = 0; l < t->rank; ++l) sz *= t->hs[(size_t)i * t->rank + l];
+      m = std::max(m, t->ho[i] + sz);
+    }
+    in_bytes += m;
+  }
+  P.bytes = 8.0 * (double)(in_bytes + P.out_total);
+  P.n_pairs = P.s1.n_pairs + P.s2.n_pairs;
+  P.n_tasks = P.s1.n_tasks + P.s2.n_tasks + P.n_mix_tasks;
+}
+
+static void run_fused(const FusedPlan &P, const double *s1A, const double *s1B, const double *W1, const double *W2,
+                      const double *s2B, double *out, int compute, hipStream_t s) {
+  if (P.n_terms == 0) {
+    if (P.out_total > 0)
+      hipLaunchKernelGGL(zero_kernel, dim3(nb256(P.out_total)), dim3(256), 0, s, P.out_total, out);
+    TNX_CHECK_HIP(hipGetLastError());
+    return;
+  }
+  hipLaunchKernelGGL(coef_kernel, dim3(nb256(P.n_terms)), dim3(256), 0, s, P.n_terms, P.cspec.p, W1, W2, P.coef.p);
+  launch_grouped_gemm(P.s1, s1A, s1B, P.X.p, compute, false, s);
+  hipLaunchKernelGGL(mix_kernel, dim3(P.n_mix_tasks), dim3(256), 0, s, P.items.p, P.mix_titem.p, P.mix_tstart.p,
+                     P.mterms.p, P.coef.p, P.X.p, P.Y.p);
+  launch_grouped_gemm(P.s2, P.Y.p, s2B, out, compute, false, s);
+  TNX_CHECK_HIP(hipGetLastError());
+}
+
+static void ensure_perblock_fused(FusedPlan &P, hipStream_t s) {
+  if (P.pb_ready) return;
+  // stage 1: tasks of each X item are contiguous in the s1 task list
+  std::vector<GemmTask> t1(P.s1.n_tasks);
+  if (P.s1.n_tasks) P.s1.tasks.download(t1.data(), P.s1.n_tasks, s);
+  TNX_CHECK_HIP(hipStreamSynchronize(s));
+  P.pb1_off.assign(P.s1.n_outs, 0);
+  P.pb1_cnt.assign(P.s1.n_outs, 0);
+  for (int k = 0; k < (int)t1.size(); ++k) {
+    const int o = t1[k].out;
+    if (P.pb1_cnt[o] == 0) P.pb1_off[o] = k;
+    P.pb1_cnt[o]++;
+  }
+  // stage 2: one single-pair program per pair
+  const auto &ho = P.s2.h_outs;
+  std::vector<GemmOut> outs(std::max(P.s2.n_pairs, 1));
+  std::vector<GemmTask> tasks;
+  P.pb2_off.assign(P.s2.n_pairs, 0);
+  P.pb2_cnt.assign(P.s2.n_pairs, 0);
+  P.pb2_first.assign(P.s2.n_pairs, 0);
+  P.pb2_zero.clear();
+  for (size_t o = 0; o < ho.size(); ++o) {
+    if (ho[o].pair_begin == ho[o].pair_end) {
+      P.pb2_zero.push_back(ho[o]);
+      continue;
+    }
+    for (int k = ho[o].pair_begin; k < ho[o].pair_end; ++k) {
+      GemmOut g = ho[o];
+      g.pair_begin = k;
+      g.pair_end = k + 1;
+      outs[k] = g;
+      P.pb2_first[k] = (k == ho[o].pair_begin);
+      P.pb2_off[k] = (int32_t)tasks.size();
+      for (int m0 = 0; m0 < g.M; m0 += GEMM_BM)
+        for (int n0 = 0; n0 < g.N; n0 += GEMM_BN) tasks.push_back(GemmTask{k, m0, n0, 0});
+      P.pb2_cnt[k] = (int32_t)tasks.size() - P.pb2_off[k];
+    }
+  }
+  P.pb2_outs.upload(outs.data(), outs.size(), s);
+  if (!tasks.empty()) P.pb2_tasks.upload(tasks.data(), tasks.size(), s);
+  TNX_CHECK_HIP(hipStreamSynchronize(s));
+  P.pb_ready = true;
+}
+
+static void run_fused_perblock(FusedPlan &P, const double *s1A, const double *s1B, const double *W1, const double *W2,
+                               const double *s2B, double *out, int compute, hipStream_t s) {
+  ensure_perblock_fused(P, s);
+  if (P.n_terms == 0) {
+    if (P.out_total > 0)
+      hipLaunchKernelGGL(zero_kernel, dim3(nb256(P.out_total)), dim3(256), 0, s, P.out_total, out);
+    return;
+  }
+  hipLaunchKernelGGL(coef_kernel, dim3(nb256(P.n_terms)), dim3(256), 0, s, P.n_terms, P.cspec.p, W1, W2, P.coef.p);
+  for (int x = 0; x < P.s1.n_outs; ++x)
+    launch_prepared(P.s1.a_lay, P.s1.b_lay, compute, P.pb1_cnt[x], s, P.s1.tasks.p + P.pb1_off[x], P.s1.outs.p,
+                    P.s1.pairs.p, s1A, s1B, P.X.p, false);
+  for (int64_t k = 0; k < P.n_items; ++k)
+    hipLaunchKernelGGL(mix_kernel, dim3(P.pbm_cnt[k]), dim3(256), 0, s, P.items.p, P.mix_titem.p + P.pbm_off[k],
+                       P.mix_tstart.p + P.pbm_off[k], P.mterms.p, P.coef.p, P.X.p, P.Y.p);
+  for (const auto &z : P.pb2_zero) {
+    const int64_t n = (int64_t)z.M * z.N;
+    if (n) hipLaunchKernelGGL(zero_kernel, dim3(nb256(n)), dim3(256), 0, s, n, out + z.c_off);
+  }
+  for (int k = 0; k < P.s2.n_pairs; ++k)
+    launch_prepared(P.s2.a_lay, P.s2.b_lay, compute, P.pb2_cnt[k], s, P.pb2_tasks.p + P.pb2_off[k], P.pb2_outs.p,
+                    P.s2.pairs.p, P.Y.p, s2B, out, !P.pb2_first[k]);
+  TNX_CHECK_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ Lanczos on the device
+constexpr int LZ_G = 256;  // reduction workgroups
+
+struct LzState {
+  int32_t stopped;
+  int32_t nsteps;
+  double inv;     // scale factor for the next normalisation
+  double energy;
+};
+
+__global__ void dot_partial_kernel(const double *__restrict__ a, const double *__restrict__ b, int64_t n,
+                                   double *__restrict__ part, const LzState *__restrict__ st) {
+  if (st && st->stopped) return;
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    acc += a[i] * b[i];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// Multi-vector dots: part[j*G + g] for j < nv (rows of V against w).
+__global__ void dot_multi_partial_kernel(const double *__restrict__ V, int64_t n, int nv, const double *__restrict__ w,
+                                         double *__restrict__ part, const LzState *__restrict__ st) {
+  if (st->stopped) return;
+  __shared__ double sh[256];
+  const int j = blockIdx.y;
+  const double *a = V + (int64_t)j * n;
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    acc += a[i] * w[i];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s2 = 128; s2 > 0; s2 >>= 1) {
+    if (threadIdx.x < s2) sh[threadIdx.x] += sh[threadIdx.x + s2];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(int64_t)j * gridDim.x + blockIdx.x] = sh[0];
+  (void)nv;
+}
+
+__device__ double block_sum(const double *part, int G) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < G; i += blockDim.x) acc += part[i];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// op 0: inv = 1/(sqrt(sum)+1e-15) (initial normalisation)
+// op 1: alpha[k] = sum
+// op 2: beta[k+1] = sqrt(sum); stop if < tol, else inv = 1/beta
+// op 3: inv = 1/(sqrt(sum)+1e-15) (final normalisation)
+__global__ void lz_finalize_kernel(const double *__restrict__ part, int G, int op, int k, double tol,
+                                   double *__restrict__ alpha, double *__restrict__ beta, LzState *__restrict__ st) {
+  if ((op == 1 || op == 2) && st->stopped) return;
+  const double s = block_sum(part, G);
+  if (threadIdx.x != 0) return;
+  if (op == 0 || op == 3) {
+    st->inv = 1.0 / (sqrt(s) + 1e-15);
+  } else if (op == 1) {
+    alpha[k] = s;
+  } else {
+    const double b = sqrt(s);
+    beta[k + 1] = b;
+    if (b < tol) {
+      st->stopped = 1;
+      st->nsteps = k + 1;
+    } else {
+      st->inv = 1.0 / b;
+    }
+  }
+}
+
+__global__ void lz_multi_finalize_kernel(const double *__restrict__ part, int G, double *__restrict__ c,
+                                         const LzState *__restrict__ st) {
+  if (st->stopped) return;
+  const double s = block_sum(part + (int64_t)blockIdx.x * G, G);
+  if (threadIdx.x == 0) c[blockIdx.x] = s;
+}
+
+// dst = src * inv  (skipped after a stop when guarded)
+__global__ void lz_scale_kernel(double *__restrict__ dst, const double *__restrict__ src, int64_t n,
+                                const LzState *__restrict__ st, int guarded) {
+  if (guarded && st->stopped) return;
+  const double f = st->inv;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i] * f;
+}
+
+// w = (w - v*alpha[k]) - vp*beta[k]   (reference order, dmrg.py:1009-1011)
+__global__ void lz_axpy_kernel(double *__restrict__ w, const double *__restrict__ v, const double *__restrict__ vp,
+                               int64_t n, int k, const double *__restrict__ alpha, const double *__restrict__ beta,
+                               const LzState *__restrict__ st) {
+  if (st->stopped) return;
+  const double a = alpha[k];
+  const double b = k > 0 ? beta[k] : 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double t = w[i] - v[i] * a;
+    if (k > 0) t = t - vp[i] * b;
+    w[i] = t;
+  }
+}
+
+// w = w - sum_j V[j]*c[j], applied sequentially in j
+__global__ void lz_reorth_kernel(double *__restrict__ w, const double *__restrict__ V, int64_t n, int nv,
+                                 const double *__restrict__ c, const LzState *__restrict__ st) {
+  if (st->stopped) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double t = w[i];
+    for (int j = 0; j < nv; ++j) t = t - V[(int64_t)j * n + i] * c[j];
+    w[i] = t;
+  }
+}
+
+// x = sum_k c[k] V[k] in order k = 0..m-1 (dmrg.py:1038-1041)
+__global__ void lz_combine_kernel(double *__restrict__ x, const double *__restrict__ V, int64_t n,
+                                  const double *__restrict__ c, const LzState *__restrict__ st) {
+  const int m = st->nsteps;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double t = V[i] * c[0];
+    for (int k = 1; k < m; ++k) t = t + V[(int64_t)k * n + i] * c[k];
+    x[i] = t;
+  }
+}
+
+__global__ void lz_copy_kernel(double *__restrict__ x, const double *__restrict__ v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = v[i];
+}
+
+// Sturm count: number of eigenvalues of T strictly below x.
+__device__ int sturm_count(const double *a, const double *b, int m, double x, double pivmin) {
+  int cnt = 0;
+  double q = a[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  if (q < 0) ++cnt;
+  for (int i = 1; i < m; ++i) {
+    q = a[i] - x - b[i] * b[i] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    if (q < 0) ++cnt;
+  }
+  return cnt;
+}
+
+// Lowest eigenpair of the m x m symmetric tridiagonal T (diag alpha[0..m),
+// off-diagonal beta[1..m)) by bisection and inverse iteration.  One thread.
+// Writes c[0..m) and st->energy; m = st->nsteps (set to max_iter when no stop).
+__global__ void lz_tridiag_kernel(const double *__restrict__ alpha, const double *__restrict__ beta,
+                                  double *__restrict__ c, double *__restrict__ work, LzState *__restrict__ st,
+                                  int max_iter) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (!st->stopped) st->nsteps = max_iter;
+  const int m = st->nsteps;
+  if (m == 1) {
+    st->energy = alpha[0];
+    c[0] = 1.0;
+    return;
+  }
+  const double *a = alpha;
+  const double *b = beta;  // b[i] couples i-1 and i, i = 1..m-1
+  double lo = 1e300, hi = -1e300, tnorm = 0.0;
+  for (int i = 0; i < m; ++i) {
+    const double r = (i > 0 ? fabs(b[i]) : 0.0) + (i + 1 < m ? fabs(b[i + 1]) : 0.0);
+    lo = fmin(lo, a[i] - r);
+    hi = fmax(hi, a[i] + r);
+    tnorm = fmax(tnorm, fabs(a[i]) + r);
+  }
+  const double eps = 2.220446049250313e-16;
+  const double pivmin = fmax(tnorm * eps * eps, 1e-300);
+  lo -= 2 * eps * tnorm + pivmin;
+  hi += 2 * eps * tnorm + pivmin;
+  for (int it = 0; it < 2000; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    if (sturm_count(a, b, m, mid, pivmin) >= 1) hi = mid; else lo = mid;
+    if (hi - lo <= 2 * eps * fmax(fabs(lo), fabs(hi))) break;
+  }
+  const double lam = 0.5 * (lo + hi);
+  st->energy = lam;
+  // inverse iteration: (T - lam) y = x, pivoted tridiagonal elimination
+  double *dl = work, *d = work + m, *du = work + 2 * m, *du2 = work + 3 * m, *y = work + 4 * m;
+  for (int i = 0; i < m; ++i) c[i] = 1.0 / sqrt((double)m);
+  const double tiny = fmax(eps * tnorm, 1e-300);
+  for (int iter = 0; iter < 4; ++iter) {
+    for (int i = 0; i < m; ++i) {
+      d[i] = a[i] - lam;
+      du[i] = (i + 1 < m) ? b[i + 1] : 0.0;
+      dl[i] = (i + 1 < m) ? b[i + 1] : 0.0;
+      du2[i] = 0.0;
+      y[i] = c[i];
+    }
+    for (int i = 0; i < m - 1; ++i) {
+      if (fabs(d[i]) >= fabs(dl[i])) {
+        if (d[i] == 0.0) d[i] = tiny;
+        const double f = dl[i] / d[i];
+        d[i + 1] -= f * du[i];
+        y[i + 1] -= f * y[i];
+        du2[i] = 0.0;
+      } else {
+        const double f = d[i] / dl[i];
+        d[i] = dl[i];
+        const double t = d[i + 1];
+        d[i + 1] = du[i] - f * t;
+        if (i < m - 2) {
+          du2[i] = du[i + 1];
+          du[i + 1] = -f * du2[i];
+        }
+        du[i] = t;
+        const double ty = y[i];
+        y[i] = y[i + 1];
+        y[i + 1] = ty - f * y[i + 1];
+      }
+    }
+    if (d[m - 1] == 0.0) d[m - 1] = tiny;
+    y[m - 1] /= d[m - 1];
+    if (m > 1) y[m - 2] = (y[m - 2] - du[m - 2] * y[m - 1]) / d[m - 2];
+    for (int i = m - 3; i >= 0; --i) y[i] = (y[i] - du[i] * y[i + 1] - du2[i] * y[i + 2]) / d[i];
+    double nrm = 0.0;
+    for (int i = 0; i < m; ++i) nrm += y[i] * y[i];
+    nrm = sqrt(nrm);
+    for (int i = 0; i < m; ++i) c[i] = y[i] / nrm;
+  }
+}
+
+struct LanczosWorkspace {
+  DevBuf<double> V;       // (max_iter + 1) x n
+  DevBuf<double> scal;    // alpha, beta, c, reorth c, work, partials
+  DevBuf<LzState> st;
+  int64_t n = 0;
+  int max_iter = 0;
+};
+
+static std::mutex g_lz_mutex;
+static LanczosWorkspace g_lz;
+
+static void lanczos_run(const FusedPlan &P, const double *L, const double *Wl, const double *Wr, const double *R,
+                        double *theta, int max_iter, double tol, int reorth, int compute, double *energy,
+                        int *nsteps, hipStream_t s) {
+  TNX_REQUIRE(max_iter >= 1, TNX_ERR_ARG, "max_iter must be >= 1");
+  const int64_t n = P.in_total;
+  std::lock_guard<std::mutex> lk(g_lz_mutex);
+  LanczosWorkspace &W = g_lz;
+  const int64_t need_v = (int64_t)(max_iter + 1) * std::max<int64_t>(n, 1);
+  if ((int64_t)W.V.n < need_v) W.V.alloc(need_v);
+  const int64_t nscal = 3 * (max_iter + 2) + (max_iter + 2) + 5 * (max_iter + 2) + (int64_t)LZ_G * (max_iter + 2);
+  if ((int64_t)W.scal.n < nscal) W.scal.alloc(nscal);
+  if (!W.st.p) W.st.alloc(1);
+  double *alpha = W.scal.p;
+  double *beta = alpha + (max_iter + 2);
+  double *cvec = beta + (max_iter + 2);
+  double *rc = cvec + (max_iter + 2);
+  double *work = rc + (max_iter + 2);
+  double *part = work + 5 * (max_iter + 2);
+  static const LzState init{0, 0, 0.0, 0.0};
+  TNX_CHECK_HIP(hipMemcpyAsync(W.st.p, &init, sizeof(LzState), hipMemcpyHostToDevice, s));
+  double *V = W.V.p;
+  const dim3 vg(std::min<int64_t>(std::max<int64_t>((n + 255) / 256, 1), 1024)), vb(256);
+  const dim3 rg(LZ_G);
+  // v0 = theta / (|theta| + 1e-15)
+  hipLaunchKernelGGL(dot_partial_kernel, rg, vb, 0, s, theta, theta, n, part, (const LzState *)nullptr);
+  hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), vb, 0, s, part, LZ_G, 0, 0, tol, alpha, beta, W.st.p);
+  hipLaunchKernelGGL(lz_scale_kernel, vg, vb, 0, s, V, theta, n, W.st.p, 0);
+  for (int k = 0; k < max_iter; ++k) {
+    double *v = V + (int64_t)k * n;
+    double *w = V + (int64_t)(k + 1) * n;
+    const double *vp = k > 0 ? V + (int64_t)(k - 1) * n : v;
+    run_fused(P, L, v, Wl, Wr, R, w, compute, s);
+    hipLaunchKernelGGL(dot_partial_kernel, rg, vb, 0, s, v, w, n, part, W.st.p);
+    hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), vb, 0, s, part, LZ_G, 1, k, tol, alpha, beta, W.st.p);
+    hipLaunchKernelGGL(lz_axpy_kernel, vg, vb, 0, s, w, v, vp, n, k, alpha, beta, W.st.p);
+    if (reorth) {
+      hipLaunchKernelGGL(dot_multi_partial_kernel, dim3(LZ_G, k + 1), vb, 0, s, V, n, k + 1, w, part, W.st.p);
+      hipLaunchKernelGGL(lz_multi_finalize_kernel, dim3(k + 1), vb, 0, s, part, LZ_G, rc, W.st.p);
+      hipLaunchKernelGGL(lz_reorth_kernel, vg, vb, 0, s, w, V, n, k + 1, rc, W.st.p);
+    }
+    hipLaunchKernelGGL(dot_partial_kernel, rg, vb, 0, s, w, w, n, part, W.st.p);
+    hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), vb, 0, s, part, LZ_G, 2, k, tol, alpha, beta, W.st.p);
+    hipLaunchKernelGGL(lz_scale_kernel, vg, vb, 0, s, w, w, n, W.st.p, 1);
+  }
+  hipLaunchKernelGGL(lz_tridiag_kernel, dim3(1), dim3(64), 0, s, alpha, beta, cvec, work, W.st.p, max_iter);
+  hipLaunchKernelGGL(lz_combine_kernel, vg, vb, 0, s, theta, V, n, cvec, W.st.p);
+  hipLaunchKernelGGL(dot_partial_kernel, rg, vb, 0, s, theta, theta, n, part, (const LzState *)nullptr);
+  hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), vb, 0, s, part, LZ_G, 3, 0, tol, alpha, beta, W.st.p);
+  hipLaunchKernelGGL(lz_scale_kernel, vg, vb, 0, s, theta, theta, n, W.st.p, 0);
+  TNX_CHECK_HIP(hipGetLastError());
+  LzState out{};
+  TNX_CHECK_HIP(hipMemcpyAsync(&out, W.st.p, sizeof(LzState), hipMemcpyDeviceToHost, s));
+  TNX_CHECK_HIP(hipStreamSynchronize(s));
+  if (out.nsteps == 1) {
+    // dmrg.py:1029-1030: a single step returns the normalised start vector
+    hipLaunchKernelGGL(lz_copy_kernel, vg, vb, 0, s, theta, V, n);
+    TNX_CHECK_HIP(hipStreamSynchronize(s));
+  }
+  *energy = out.energy;
+  *nsteps = out.nsteps;
+}
+
+}  // namespace tnx
+
+using namespace tnx;
+
+extern "C" {
+
+int tnx_matvec_plan(const tnx_blocks *L, const tnx_blocks *theta, const tnx_blocks *Wl, const tnx_blocks *Wr,
+                    const tnx_blocks *R, void *stream, tnx_plan **plan) {
+  return guarded([&] {
+    TNX_REQUIRE(plan, TNX_ERR_ARG, "null plan pointer");
+    hipStream_t s = (hipStream_t)stream;
+    DevTable tL, tT, tWl, tWr, tR;
+    tL.up(L, 3, "L", s);
+    tT.up(theta, 4, "theta", s);
+    tWl.up(Wl, 4, "Wl", s);
+    tWr.up(Wr, 4, "Wr", s);
+    tR.up(R, 3, "R", s);
+    for (int i = 0; i < tT.n; ++i)
+      TNX_REQUIRE(tT.hs[i * 4 + 1] == 1 && tT.hs[i * 4 + 2] == 1, TNX_ERR_BAD_STRUCTURE,
+                  "fused matvec needs physical legs of multiplicity one");
+    std::unique_ptr<FusedPlan> P(new FusedPlan());
+    build_fused(*P, 0, tL, tT, tWl, &tWr, tR, &tT, nullptr, s);
+    *plan = P.release();
+  });
+}
+
+int tnx_matvec_reach(const tnx_plan *plan, int32_t *n_edges, int32_t *src, int32_t *dst, int32_t capacity) {
+  return guarded([&] {
+    TNX_REQUIRE(plan && plan->kind == PlanKind::Matvec, TNX_ERR_ARG, "not a matvec plan");
+    const FusedPlan &P = static_cast<const FusedPlan &>(*plan);
+    const int32_t ne = (int32_t)P.reach_src.size();
+    if (n_edges) *n_edges = ne;
+    if (ne > capacity) throw Error{TNX_ERR_CAPACITY, "reach capacity too small"};
+    if (ne) {
+      std::memcpy(src, P.reach_src.data(), ne * sizeof(int32_t));
+      std::memcpy(dst, P.reach_dst.data(), ne * sizeof(int32_t));
+    }
+  });
+}
+
+int tnx_matvec_execute(const tnx_plan *plan, const double *L, const double *Wl, const double *Wr, const double *R,
+                       const double *x, double *y, int32_t compute, void *stream) {
+  return guarded([&] {
+    TNX_REQUIRE(plan && plan->kind == PlanKind::Matvec, TNX_ERR_ARG, "not a matvec plan");
+    const FusedPlan &P = static_cast<const FusedPlan &>(*plan);
+    run_fused(P, L, x, Wl, Wr, R, y, compute, (hipStream_t)stream);
+  });
+}
+
+int tnx_matvec_execute_stage(const tnx_plan *plan, int32_t stage, const double *L, const double *Wl,
+                             const double *Wr, const double *R, const double *x, double *y, int32_t compute,
+                             void *stream) {
+  return guarded([&] {
+    TNX_REQUIRE(plan && plan->kind == PlanKind::Matvec, TNX_ERR_ARG, "not a matvec plan");
+    const FusedPlan &P = static_cast<const FusedPlan &>(*plan);
+    hipStream_t s = (hipStream_t)stream;
+    if (P.n_terms == 0) return;
+    switch (stage) {
+      case 0:
+        hipLaunchKernelGGL(coef_kernel, dim3(nb256(P.n_terms)), dim3(256), 0, s, P.n_terms, P.cspec.p, Wl, Wr,
+                           P.coef.p);
+        break;
+      case 1:
+        launch_grouped_gemm(P.s1, L, x, P.X.p, compute, false, s);
+        break;
+      case 2:
+        hipLaunchKernelGGL(mix_kernel, dim3(P.n_mix_tasks), dim3(256), 0, s, P.items.p, P.mix_titem.p,
+                           P.mix_tstart.p, P.mterms.p, P.coef.p, P.X.p, P.Y.p);
+        break;
+      case 3:
+        launch_grouped_gemm(P.s2, P.Y.p, R, y, compute, false, s);
+        break;
+      default:
+        throw Error{TNX_ERR_ARG, "stage must be 0..3"};
+    }
+    TNX_CHECK_HIP(hipGetLastError());
+  });
+}
+
+int tnx_fused_stage_flops(const tnx_plan *plan, double *flops4) {
+  return guarded([&] {
+    TNX_REQUIRE(plan && flops4, TNX_ERR_ARG, "null argument");
+    TNX_REQUIRE(plan->kind == PlanKind::Matvec || plan->kind == PlanKind::EnvLeft ||
+                    plan->kind == PlanKind::EnvRight,
+                TNX_ERR_ARG, "not a fused plan");
+    const FusedPlan &P = static_cast<const FusedPlan &>(*plan);
+    flops4[0] = 0.0;
+    flops4[1] = P.s1.flops;
+    flops4[2] = P.mix_flops;
+    flops4[3] = P.s2.flops;
+  });
+}
+
+int tnx_matvec_execute_perblock(const tnx_plan *plan, const double *L, const double *Wl, const double *Wr,
+                                const double *R, const double *x, double *y, int32_t compute, void *stream) {
+  return guarded([&] {
+    TNX_REQUIRE(plan && plan->kind == PlanKind::Matvec, TNX_ERR_ARG, "not a matvec plan");
+    FusedPlan &P = const_cast<FusedPlan &>(static_cast<const FusedPlan &>(*plan));
+    run_fused_perblock(P, L, x, Wl, Wr, R, y, compute, (hipStream_t)stream);
+  });
+}
+
+int tnx_env_plan(int32_t side, const tnx_blocks *E, const tnx_blocks *A, const tnx_blocks *W,
+                 const tnx_leg *out_legs, void *stream, tnx_plan **plan) {
+  return guarded([&] {
+    TNX_REQUIRE(plan && out_legs, TNX_ERR_ARG, "null argument");
+    TNX_REQUIRE(side == 0, TNX_ERR_ARG,
+                "only left-form environment plans exist: evaluate a right environment as the left form with the "
+                "site transposed to (a,p,d) and the MPO tensor to (b,p,x,e)");
+    hipStream_t s = (hipStream_t)stream;
+    DevTable tE, tA, tW;
+    tE.up(E, 3, "E", s);
+    tA.up(A, 3, "A", s);
+    tW.up(W, 4, "W", s);
+    for (int i = 0; i < tA.n; ++i)
+      TNX_REQUIRE(tA.hs[i * 3 + 1] == 1, TNX_ERR_BAD_STRUCTURE, "environment plans need physical multiplicity one");
+    std::unique_ptr<FusedPlan> P(new FusedPlan());
+    build_fused(*P, 1, tE, tA, tW, nullptr, tA, nullptr, out_legs, s);
+    *plan = P.release();
+  });
+}
+
+int tnx_env_execute(const tnx_plan *plan, const double *E, const double *A, const double *W, double *out,
+                    int32_t compute, void *stream) {
+  return guarded([&] {
+    TNX_REQUIRE(plan && (plan->kind == PlanKind::EnvLeft || plan->kind == PlanKind::EnvRight), TNX_ERR_ARG,
+                "not an environment plan");
+    const FusedPlan &P = static_cast<const FusedPlan &>(*plan);
+    hipStream_t s = (hipStream_t)stream;
+    run_fused(P, E, A, W, nullptr, A, out, compute, s);
+  });
+}
+
+int tnx_lanczos(const tnx_plan *matvec_plan, const double *L, const double *Wl, const double *Wr, const double *R,
+                double *theta, int32_t max_iter, double tol, int32_t reorth, int32_t compute, double *energy,
+                int32_t *n_steps, void *stream) {
+  return guarded([&] {
+    TNX_REQUIRE(matvec_plan && matvec_plan->kind == PlanKind::Matvec, TNX_ERR_ARG, "not a matvec plan");
+    TNX_REQUIRE(energy && n_steps, TNX_ERR_ARG, "null output");
+    const FusedPlan &P = static_cast<const FusedPlan &>(*matvec_plan);
+    int ns = 0;
+    lanczos_run(P, L, Wl, Wr, R, theta, max_iter, tol, reorth, compute, energy, &ns, (hipStream_t)stream);
+    *n_steps = ns;
+  });
+}
+
+}  // extern "C"

@@ -42,26 +42,18 @@ __device__ __forceinline__ void load_a(const double *__restrict__ A, const GemmP
   }
 }
 
-template <int BLAY>
-__device__ __forceinline__ void load_b(const double *__restrict__ B, const GemmPair &p, int N, int n0,
-                                       int k0, int tid, double (&r)[4]) {
-  if (BLAY == LAY_KN) {
-    const int k = k0 + (tid >> 4);
-    const int nq = n0 + ((tid & 15) << 2);
-    const double *src = B + p.b_off + (int64_t)k * p.ldb;
+// B operand is always K x N row-major (KN): b[k*ldb + n].
+__device__ __forceinline__ void load_b(const double *__restrict__ B, const GemmPair &p, int N, int n0, int k0,
+                                       int tid, double (&r)[4]) {
+  const int k = k0 + (tid >> 4);
+  const int nq = n0 + ((tid & 15) << 2);
+  const double *src = B + p.b_off + (int64_t)k * p.ldb;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) r[i] = (k < p.K && nq + i < N) ? src[nq + i] : 0.0;
-  } else {
-    const int n = n0 + (tid >> 2);
-    const int kq = k0 + ((tid & 3) << 2);
-    const double *src = B + p.b_off + (int64_t)n * p.ldb;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) r[i] = (n < N && kq + i < p.K) ? src[kq + i] : 0.0;
-  }
+  for (int i = 0; i < 4; ++i) r[i] = (k < p.K && nq + i < N) ? src[nq + i] : 0.0;
 }
 
 // Store staged registers into LDS tiles laid out [k][m] and [k][n].
-template <int ALAY, int BLAY, class T>
+template <int ALAY, class T>
 __device__ __forceinline__ void store_tiles(T *As, T *Bs, const Stage &st, int tid) {
   if (ALAY == LAY_MK) {
     const int m = tid >> 2, kq = (tid & 3) << 2;
@@ -72,19 +64,15 @@ __device__ __forceinline__ void store_tiles(T *As, T *Bs, const Stage &st, int t
 #pragma unroll
     for (int i = 0; i < 4; ++i) As[k * APITCH + mq + i] = (T)st.a[i];
   }
-  if (BLAY == LAY_KN) {
+  {
     const int k = tid >> 4, nq = (tid & 15) << 2;
 #pragma unroll
     for (int i = 0; i < 4; ++i) Bs[k * BPITCH + nq + i] = (T)st.b[i];
-  } else {
-    const int n = tid >> 2, kq = (tid & 3) << 2;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) Bs[(kq + i) * BPITCH + n] = (T)st.b[i];
   }
 }
 
 // Split-FP32 staging: hi = fp32(x), lo = fp32(x - hi).
-template <int ALAY, int BLAY>
+template <int ALAY>
 __device__ __forceinline__ void store_tiles_split(float *Ah, float *Al, float *Bh, float *Bl,
                                                   const Stage &st, int tid) {
   float ah[4], al[4], bh[4], bl[4];
@@ -110,19 +98,12 @@ __device__ __forceinline__ void store_tiles_split(float *Ah, float *Al, float *B
       Al[k * APITCH + mq + i] = al[i];
     }
   }
-  if (BLAY == LAY_KN) {
+  {
     const int k = tid >> 4, nq = (tid & 15) << 2;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       Bh[k * BPITCH + nq + i] = bh[i];
       Bl[k * BPITCH + nq + i] = bl[i];
-    }
-  } else {
-    const int n = tid >> 2, kq = (tid & 3) << 2;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      Bh[(kq + i) * BPITCH + n] = bh[i];
-      Bl[(kq + i) * BPITCH + n] = bl[i];
     }
   }
 }
@@ -148,7 +129,7 @@ __device__ __forceinline__ void store_acc(double *__restrict__ C, const GemmOut 
 }
 
 // COMPUTE: 0 = FP64 MFMA, 1 = FP32 MFMA, 2 = split FP32 (3 products, FP64 across K chunks)
-template <int ALAY, int BLAY, int COMPUTE>
+template <int ALAY, int COMPUTE>
 __global__ __launch_bounds__(GEMM_THREADS) void grouped_gemm_kernel(
     const GemmTask *__restrict__ tasks, const GemmOut *__restrict__ outs,
     const GemmPair *__restrict__ pairs, const double *__restrict__ A, const double *__restrict__ B,
@@ -187,14 +168,14 @@ __global__ __launch_bounds__(GEMM_THREADS) void grouped_gemm_kernel(
     GemmPair p = pairs[pi];
     Stage st;
     load_a<ALAY>(A, p, o.M, t.m0, k0, tid, st.a);
-    load_b<BLAY>(B, p, o.N, t.n0, k0, tid, st.b);
+    load_b(B, p, o.N, t.n0, k0, tid, st.b);
     while (true) {
       __syncthreads();
       if (COMPUTE == 2)
-        store_tiles_split<ALAY, BLAY>((float *)As[0], (float *)As[NT - 1], (float *)Bs[0],
+        store_tiles_split<ALAY>((float *)As[0], (float *)As[NT - 1], (float *)Bs[0],
                                       (float *)Bs[NT - 1], st, tid);
       else
-        store_tiles<ALAY, BLAY, T>(As[0], Bs[0], st, tid);
+        store_tiles<ALAY, T>(As[0], Bs[0], st, tid);
       __syncthreads();
       // prefetch next tile into registers
       int nk0 = k0 + GEMM_BK;
@@ -208,7 +189,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void grouped_gemm_kernel(
       const bool more = npi < o.pair_end;
       if (more) {
         load_a<ALAY>(A, np, o.M, t.m0, nk0, tid, st.a);
-        load_b<BLAY>(B, np, o.N, t.n0, nk0, tid, st.b);
+        load_b(B, np, o.N, t.n0, nk0, tid, st.b);
       }
 #pragma unroll
       for (int kk = 0; kk < GEMM_BK; kk += 4) {
@@ -286,21 +267,21 @@ __global__ __launch_bounds__(GEMM_THREADS) void grouped_gemm_kernel(
     }
 }
 
-template <int ALAY, int BLAY>
+template <int ALAY>
 static void launch_lay(int compute, dim3 grid, hipStream_t s, const GemmTask *tasks, const GemmOut *outs,
                        const GemmPair *pairs, const double *A, const double *B, double *C, int acc) {
   switch (compute) {
     case TNX_COMPUTE_F64:
-      hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, BLAY, 0>), grid, dim3(GEMM_THREADS), 0, s, tasks, outs,
-                         pairs, A, B, C, acc);
+      hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, 0>), grid, dim3(GEMM_THREADS), 0, s, tasks, outs, pairs, A, B,
+                         C, acc);
       break;
     case TNX_COMPUTE_F32:
-      hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, BLAY, 1>), grid, dim3(GEMM_THREADS), 0, s, tasks, outs,
-                         pairs, A, B, C, acc);
+      hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, 1>), grid, dim3(GEMM_THREADS), 0, s, tasks, outs, pairs, A, B,
+                         C, acc);
       break;
     case TNX_COMPUTE_SPLIT_F32:
-      hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, BLAY, 2>), grid, dim3(GEMM_THREADS), 0, s, tasks, outs,
-                         pairs, A, B, C, acc);
+      hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, 2>), grid, dim3(GEMM_THREADS), 0, s, tasks, outs, pairs, A, B,
+                         C, acc);
       break;
     default:
       throw Error{TNX_ERR_DTYPE, "unknown compute mode " + std::to_string(compute)};
@@ -311,15 +292,12 @@ static void launch_any(int a_lay, int b_lay, int compute, int n_tasks, hipStream
                        const GemmOut *outs, const GemmPair *pairs, const double *A, const double *B, double *C,
                        int acc) {
   if (n_tasks <= 0) return;
+  TNX_REQUIRE(b_lay == LAY_KN, TNX_ERR_ARG, "B operands must be K x N row-major");
   dim3 grid(n_tasks);
-  if (a_lay == LAY_MK && b_lay == LAY_KN)
-    launch_lay<LAY_MK, LAY_KN>(compute, grid, s, tasks, outs, pairs, A, B, C, acc);
-  else if (a_lay == LAY_MK && b_lay == LAY_NK)
-    launch_lay<LAY_MK, LAY_NK>(compute, grid, s, tasks, outs, pairs, A, B, C, acc);
-  else if (a_lay == LAY_KM && b_lay == LAY_KN)
-    launch_lay<LAY_KM, LAY_KN>(compute, grid, s, tasks, outs, pairs, A, B, C, acc);
+  if (a_lay == LAY_MK)
+    launch_lay<LAY_MK>(compute, grid, s, tasks, outs, pairs, A, B, C, acc);
   else
-    launch_lay<LAY_KM, LAY_NK>(compute, grid, s, tasks, outs, pairs, A, B, C, acc);
+    launch_lay<LAY_KM>(compute, grid, s, tasks, outs, pairs, A, B, C, acc);
   TNX_CHECK_HIP(hipGetLastError());
 }
 

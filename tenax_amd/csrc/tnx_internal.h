@@ -119,7 +119,7 @@ struct GemmTask {
 };
 
 enum : int { LAY_MK = 0, LAY_KM = 1 };  // A: MK = a[m*lda+k], KM = a[k*lda+m]
-enum : int { LAY_KN = 0, LAY_NK = 1 };  // B: KN = b[k*ldb+n], NK = b[n*ldb+k]
+enum : int { LAY_KN = 0 };              // B: KN = b[k*ldb+n] (the only B layout)
 
 constexpr int GEMM_BM = 64;
 constexpr int GEMM_BN = 64;
@@ -150,6 +150,18 @@ void launch_prepared(int a_lay, int b_lay, int compute, int n_tasks, hipStream_t
 // Device-side tiling of a program whose pairs/outs are already on the device:
 // builds the tile list (longest K first) and downloads host mirrors.
 void finalize_program(GemmProgram &prog, hipStream_t s);
+
+// ---------------------------------------------------------------- sector enumeration
+// Sorted key set on the device (keys/shapes: n x rank, lexicographic order).
+struct KeySet {
+  int32_t rank = 0;
+  int64_t n = 0;
+  DevBuf<int32_t> keys, shapes;
+};
+// all_tuples: every charge tuple (reachability candidates) instead of the
+// conservation-filtered valid set.
+void enumerate_keys(const tnx_leg *legs, int rank, int sym_n, bool all_tuples, bool has_target, int target,
+                    hipStream_t s, KeySet &out);
 
 // ---------------------------------------------------------------- plans
 enum class PlanKind { Pairwise, Transpose, Matvec, EnvLeft, EnvRight };

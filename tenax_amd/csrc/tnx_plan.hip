@@ -175,16 +175,9 @@ __global__ void flag_to_i64_kernel(int64_t n, const int32_t *__restrict__ f, int
   if (i < n) o[i] = f[i];
 }
 
-// Sorted key set on the device.
-struct KeySet {
-  int32_t rank = 0;
-  int64_t n = 0;
-  DevBuf<int32_t> keys, shapes;
-};
-
 static constexpr int64_t MAX_CANDIDATES = int64_t(1) << 28;
 
-static void enumerate_keys(const tnx_leg *legs, int rank, int sym_n, bool all_tuples, bool has_target,
+void enumerate_keys(const tnx_leg *legs, int rank, int sym_n, bool all_tuples, bool has_target,
                            int target, hipStream_t s, KeySet &out) {
   TNX_REQUIRE(rank >= 1 && rank <= MAX_RANK, TNX_ERR_ARG, "rank out of range");
   TNX_REQUIRE(sym_n >= 0, TNX_ERR_ARG, "sym_n must be >= 0");
@@ -338,7 +331,8 @@ __global__ void build_pairs_kernel(int64_t n, PairSpec sp, int a_kfirst, int b_k
   g.b_off = offB[j];
   g.K = (int32_t)K;
   g.lda = a_kfirst ? (int32_t)M : (int32_t)K;
-  g.ldb = b_kfirst ? (int32_t)N : (int32_t)K;
+  g.ldb = (int32_t)N;
+  (void)b_kfirst;
   g.pad = 0;
   pairs[k] = g;
 }
@@ -472,7 +466,8 @@ static void build_pairwise(PairwisePlan &P, const tnx_blocks *a, const tnx_block
   P.kind = PlanKind::Pairwise;
   P.out_rank = rOut;
   P.prog.a_lay = a_kfirst ? LAY_KM : LAY_MK;
-  P.prog.b_lay = b_kfirst ? LAY_KN : LAY_NK;
+  TNX_REQUIRE(b_kfirst == 1, TNX_ERR_ARG, "B operand legs must be [contracted..., free...] (b_kfirst = 1)");
+  P.prog.b_lay = LAY_KN;
   for (int i = 0; i < a->n_blocks; ++i) {
     int64_t sz = 1;
     for (int l = 0; l < a->rank; ++l) sz *= a->shapes[i * a->rank + l];

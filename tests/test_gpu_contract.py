@@ -172,7 +172,7 @@ def test_from_dense_roundtrip_and_error(gpu):
     assert_same_structure(T, o)
     np.testing.assert_array_equal(T.todense().cpu().numpy(), d)
     bad = d.copy()
-    bad[0, 0, 0] = 1.0  # (q=-1, p=+1, q=0): violates conservation
+    bad[0, 1, 0] = 1.0  # (q=-1, p=-1, q=0): -1 - 1 - 0 != 0 violates conservation
     with pytest.raises(ValueError, match="outside symmetry-allowed"):
         SymmetricTensor.from_dense(bad, product_indices(legs))
 

@@ -1,0 +1,166 @@
+"""GPU parity of the fused DMRG kernels and the device Lanczos against the FP64 oracle."""
+
+import importlib
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import contract as ocn
+from oracle import dmrg as od
+from tests.helpers import assert_same_structure, from_oracle, max_rel_err, to_oracle
+
+pytestmark = pytest.mark.gpu
+
+ED = {(4, 0): -1.616025403784438, (6, 0): -2.493577133887925, (4, 2): -0.957106781186548}
+
+
+def _oracle_state(L, chi, seed, sweeps=1):
+    """An oracle MPS after a few oracle sweeps (nontrivial block structure)."""
+    mpo = od.build_mpo_heisenberg(L)
+    mps = od.build_random_symmetric_mps(L, bond_dim=4, seed=seed)
+    if sweeps:
+        cfg = od.DMRGConfig(max_bond_dim=chi, num_sweeps=sweeps, lanczos_max_iter=8)
+        _, _, mps, _, _ = od.dmrg(mpo, mps, cfg)
+    return mpo, mps
+
+
+@pytest.fixture(scope="module")
+def state8():
+    return _oracle_state(8, 12, 3)
+
+
+def test_env_updates_match_oracle(gpu, state8):
+    pd = importlib.import_module("tenax_amd.algorithms.dmrg")
+
+    mpo, mps = state8
+    L = len(mpo)
+    oL = od.trivial_left_env()
+    pL = pd._build_trivial_left_env_symmetric()
+    for i in range(L - 1):
+        oL = od.update_left_env(oL, mps[i], mpo[i])
+        pL = pd._update_left_env_symmetric(pL, from_oracle(mps[i]), from_oracle(mpo[i]))
+        assert_same_structure(pL, oL)
+        assert max_rel_err(pL, oL) < 1e-12
+    oR = od.trivial_right_env()
+    pR = pd._build_trivial_right_env_symmetric()
+    for i in range(L - 1, 0, -1):
+        oR = od.update_right_env(oR, mps[i], mpo[i])
+        pR = pd._update_right_env_symmetric(pR, from_oracle(mps[i]), from_oracle(mpo[i]))
+        assert_same_structure(pR, oR)
+        assert max_rel_err(pR, oR) < 1e-12
+
+
+def _problem(mpo, mps, i):
+    pd = importlib.import_module("tenax_amd.algorithms.dmrg")
+
+    L = len(mpo)
+    oL = od.trivial_left_env()
+    for k in range(i):
+        oL = od.update_left_env(oL, mps[k], mpo[k])
+    oR = od.trivial_right_env()
+    for k in range(L - 1, i + 1, -1):
+        oR = od.update_right_env(oR, mps[k], mpo[k])
+    oth = ocn.contract(mps[i], mps[i + 1])
+    pad = None
+    if oth.ndim == 3:
+        pad = not oth.labels()[0].startswith("v")
+        oth = od.pad(oth, pad)
+    P = from_oracle
+    theta, nd, pl = pd._theta_padded(P(mps[i]), P(mps[i + 1]))
+    prob = pd.TwoSiteProblem(theta, P(oL), P(mpo[i]), P(mpo[i + 1]), P(oR))
+    return prob, (oL, oth, mpo[i], mpo[i + 1], oR)
+
+
+@pytest.mark.parametrize("site", [0, 3, 6])
+def test_fused_matvec_matches_oracle(gpu, state8, site):
+    pd = importlib.import_module("tenax_amd.algorithms.dmrg")
+
+    mpo, mps = state8
+    prob, (oL, oth, oWl, oWr, oR) = _problem(mpo, mps, site)
+    ref = od.matvec(oL, oth, oWl, oWr, oR)
+    y = prob.matvec(prob.x, 0)
+    prob2 = prob
+    out = pd.SymmetricTensor._wrap(prob.indices, prob.keys, prob.shapes, prob.offsets, y)
+    po = to_oracle(out)
+    for k, b in ref.blocks.items():
+        np.testing.assert_allclose(po.blocks[k], b, atol=1e-12 * max(1.0, np.abs(b).max()))
+    for k, b in po.blocks.items():
+        if k not in ref.blocks:
+            assert np.all(b == 0.0)
+    # reach edges reproduce the reference's output key set
+    pres = prob2.present_after(2)
+    got = {tuple(int(q) for q in prob.keys[j]) for j in np.nonzero(pres)[0]}
+    assert got == set(ref.blocks) | set(oth.blocks)
+    # per-block path gives the same numbers
+    y2 = prob.matvec(prob.x, 0, per_block=True)
+    assert (y - y2).abs().max().item() <= 1e-12 * y.abs().max().item()
+
+
+@pytest.mark.parametrize("mode,tol", [("f32", 1e-5), ("split_f32", 1e-6)])
+def test_fused_matvec_reduced_precision(gpu, state8, mode, tol):
+    from tenax_amd import _lib
+
+    mpo, mps = state8
+    prob, _ = _problem(mpo, mps, 3)
+    y64 = prob.matvec(prob.x, 0)
+    y = prob.matvec(prob.x, _lib.COMPUTE[mode])
+    err = (y - y64).abs().max().item() / y64.abs().max().item()
+    assert err < tol, err
+
+
+@pytest.mark.parametrize("reorth", [False, True])
+def test_device_lanczos_matches_oracle(gpu, state8, reorth):
+    mpo, mps = state8
+    prob, (oL, oth, oWl, oWr, oR) = _problem(mpo, mps, 3)
+    e_ref, v_ref = od.lanczos(lambda v: od.matvec(oL, v, oWl, oWr, oR), oth, 30, 1e-12, reorth)
+    e, n = prob.lanczos(30, 1e-12, reorth, 0)
+    assert n == 30
+    assert abs(e - e_ref) < 1e-10
+    got = prob.extract(prob.present_after(n))
+    assert sorted(tuple(int(q) for q in k) for k in got.keys) == sorted(v_ref.blocks)
+    # Ritz vectors agree up to sign
+    po = to_oracle(got)
+    dot = sum(float(np.sum(po.blocks[k] * v_ref.blocks[k])) for k in v_ref.blocks)
+    assert abs(abs(dot) - 1.0) < 1e-8
+
+
+@pytest.mark.parametrize("L,tgt", [(4, 0), (6, 0), (4, 2)])
+def test_dmrg_matches_ed(gpu, L, tgt):
+    from tenax_amd import DMRGConfig, build_mpo_heisenberg, build_random_symmetric_mps, dmrg
+
+    mpo = build_mpo_heisenberg(L)
+    mps = build_random_symmetric_mps(L, bond_dim=4, seed=7, target_charge=tgt)
+    res = dmrg(mpo, mps, DMRGConfig(max_bond_dim=16, num_sweeps=6, lanczos_max_iter=30,
+                                    target_charge=tgt))
+    assert abs(res.energy - ED[(L, tgt)]) < 1e-10
+
+
+def test_dmrg_matches_oracle_trajectory(gpu):
+    """Same initial tensors, same schedule: energies per sweep vs the FP64 oracle."""
+    from tenax_amd import DMRGConfig, build_mpo_heisenberg, build_random_symmetric_mps, dmrg
+
+    L, chi = 10, 16
+    cfg = dict(max_bond_dim=chi, num_sweeps=3, lanczos_max_iter=20)
+    e_o, es_o, mps_o, errs_o, _ = od.dmrg(od.build_mpo_heisenberg(L), od.build_random_symmetric_mps(L, 4, seed=0),
+                                          od.DMRGConfig(**cfg))
+    res = dmrg(build_mpo_heisenberg(L), build_random_symmetric_mps(L, 4, seed=0), DMRGConfig(**cfg))
+    assert np.max(np.abs(np.array(res.energies_per_sweep) - np.array(es_o))) < 1e-10
+    for i in range(L):
+        t = res.mps.get_tensor(i)
+        o = mps_o[i]
+        assert sorted(tuple(int(q) for q in k) for k in t.keys) == sorted(o.blocks)
+
+
+def test_dmrg_rejects_mixed_and_dense(gpu):
+    from tenax_amd import DMRGConfig, TensorNetwork, build_mpo_heisenberg, build_random_symmetric_mps, dmrg
+    from tenax_amd.core.tensor import DenseTensor
+
+    mpo = build_mpo_heisenberg(4)
+    mps = build_random_symmetric_mps(4, bond_dim=4, seed=0)
+    dense = TensorNetwork()
+    for i in range(4):
+        t = mpo.get_tensor(i)
+        dense.add_node(i, DenseTensor(t.todense(), t.indices))
+    with pytest.raises(TypeError, match="uniform tensor types"):
+        dmrg(dense, mps, DMRGConfig(max_bond_dim=8, num_sweeps=1))
