@@ -124,6 +124,56 @@ def cpu_baseline(prob, mpo_pair):
     return dt, num / den
 
 
+def run_sweep(pd, mps, mpo, chi, per_block_bonds=2):
+    """One DMRG sweep (L->R, rebuild right environments, R->L; dmrg.py:179-265) on the chi state.
+
+    Returns (sweep seconds, phase breakdown, fused and per-block seconds over the
+    same `per_block_bonds` two-site updates at the centre).
+    """
+    import torch
+
+    from tenax_amd import DMRGConfig
+
+    L = len(mpo)
+    cfg = DMRGConfig(max_bond_dim=chi, num_sweeps=1, lanczos_max_iter=50)
+    ops = pd._symmetric_ops()
+    mps = list(mps)
+    left = pd._build_left_environments_list(mps, mpo, L, ops)
+    right = pd._build_right_environments_list(mps, mpo, L, ops)
+    # fused vs per-block over the same updates (no SVD, same inputs)
+    c = L // 2 - 1
+    cmp = {}
+    for name, fn in (("fused", ops.two_site_update), ("per_block", pd._per_block_two_site)):
+        torch.cuda.synchronize()
+        t0 = time.time()
+        for i in range(c, c + per_block_bonds):
+            fn(mps[i], mps[i + 1], left[i], mpo[i], mpo[i + 1], right[i + 2], cfg)
+        torch.cuda.synchronize()
+        cmp[name] = time.time() - t0
+    pd.TIMERS.clear()
+    pd.TIMERS["enabled"] = True
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for i in range(L - 1):
+        r = right[i + 2] if right[i + 2] is not None else ops.build_trivial_right_env()
+        th, e = ops.two_site_update(mps[i], mps[i + 1], left[i], mpo[i], mpo[i + 1], r, cfg)
+        A, s, B, err = pd._svd_and_truncate_site(th, i, cfg)
+        mps[i], mps[i + 1] = A, B
+        left[i + 1] = ops.update_left_env(left[i], A, mpo[i])
+    right = pd._build_right_environments_list(mps, mpo, L, ops)
+    for i in range(L - 2, -1, -1):
+        r2 = right[i + 2] if right[i + 2] is not None else ops.build_trivial_right_env()
+        th, e = ops.two_site_update(mps[i], mps[i + 1], left[i], mpo[i], mpo[i + 1], r2, cfg)
+        A, s, B, err = pd._svd_and_truncate_site(th, i, cfg, sweep_right=False)
+        mps[i], mps[i + 1] = A, B
+        right[i + 1] = ops.update_right_env(r2, B, mpo[i + 1])
+    torch.cuda.synchronize()
+    sweep = time.time() - t0
+    phases = {k: v for k, v in pd.TIMERS.items() if k != "enabled"}
+    pd.TIMERS["enabled"] = False
+    return sweep, phases, cmp
+
+
 def run_c1_parity():
     """C1 (L=20, chi=64, 10 sweeps, seed 0) on the GPU vs the FP64 oracle's golden trajectory."""
     from tenax_amd import DMRGConfig, build_mpo_heisenberg, build_random_symmetric_mps, dmrg
@@ -153,6 +203,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--perblock-steps", type=int, default=2)
+    ap.add_argument("--sweep", action="store_true", help="also time one full DMRG sweep on the chi state")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -235,6 +286,14 @@ def main():
         "lanczos_50_steps_s": lz_s,
         "sweep_s": None,
     }
+    if args.sweep:
+        sw, phases, cmp = run_sweep(pd, mps, mpo, args.chi)
+        result["sweep_s"] = sw
+        result["sweep_phases_s"] = phases
+        result["two_site_updates_centre_s"] = {
+            "bonds": 2, "fused": cmp["fused"], "per_block": cmp["per_block"],
+            "speedup_fused": cmp["per_block"] / cmp["fused"]}
+        log(f"sweep {sw:.2f}s phases {phases} updates {cmp}")
     if not args.no_cpu:
         dt, err = cpu_baseline(prob, (Wl, Wr))
         cores = os.cpu_count()

@@ -188,16 +188,43 @@ def _env_update(side: int, env: SymmetricTensor, site: SymmetricTensor, mpo: Sym
 def _update_left_env_symmetric(left_env, mps_site, mpo_site) -> SymmetricTensor:
     """new_L[d,e,f] = L[a,b,c] A[a,p,d] W[b,p,x,e] A*[c,x,f]  (dmrg.py:1165-1198)."""
     _assert_symmetric(left_env, mps_site, mpo_site, context="_update_left_env_symmetric")
-    return _env_update(0, left_env, mps_site, mpo_site, _ACTIVE_CONFIG.get("cfg"))
+    with _phase("env"):
+        return _env_update(0, left_env, mps_site, mpo_site, _ACTIVE_CONFIG.get("cfg"))
 
 
 def _update_right_env_symmetric(right_env, mps_site, mpo_site) -> SymmetricTensor:
     """new_R[d,e,f] = R[a,b,c] B[d,p,a] W[e,p,x,b] B*[f,x,c]  (dmrg.py:1201-1232)."""
     _assert_symmetric(right_env, mps_site, mpo_site, context="_update_right_env_symmetric")
-    return _env_update(1, right_env, mps_site, mpo_site, _ACTIVE_CONFIG.get("cfg"))
+    with _phase("env"):
+        return _env_update(1, right_env, mps_site, mpo_site, _ACTIVE_CONFIG.get("cfg"))
 
 
 _ACTIVE_CONFIG: dict = {}
+
+# Optional wall-clock accounting of the sweep phases (device-synchronised at
+# phase boundaries while enabled; used by the benchmark).
+TIMERS: dict = {"enabled": False}
+
+
+class _phase:
+    def __init__(self, name):
+        self.name = name
+
+    def __enter__(self):
+        if TIMERS["enabled"]:
+            import time
+
+            torch.cuda.synchronize()
+            self.t0 = time.perf_counter()
+        return self
+
+    def __exit__(self, *exc):
+        if TIMERS["enabled"]:
+            import time
+
+            torch.cuda.synchronize()
+            TIMERS[self.name] = TIMERS.get(self.name, 0.0) + time.perf_counter() - self.t0
+        return False
 
 
 def _uniform_target(t: SymmetricTensor):
@@ -289,10 +316,13 @@ def _theta_padded(site_l, site_r):
 def _two_site_update_symmetric(site_l, site_r, left_env, mpo_l, mpo_r, right_env, config):
     """dmrg.py:1235-1299 with the device Lanczos."""
     _assert_symmetric(site_l, site_r, left_env, mpo_l, mpo_r, right_env, context="_two_site_update_symmetric")
-    theta, nd, pad_left = _theta_padded(site_l, site_r)
-    prob = TwoSiteProblem(theta, left_env, mpo_l, mpo_r, right_env)
-    energy, n_steps = prob.lanczos(config.lanczos_max_iter, config.lanczos_tol, config.lanczos_reorth,
-                                   _compute(config))
+    with _phase("theta"):
+        theta, nd, pad_left = _theta_padded(site_l, site_r)
+    with _phase("plan"):
+        prob = TwoSiteProblem(theta, left_env, mpo_l, mpo_r, right_env)
+    with _phase("lanczos"):
+        energy, n_steps = prob.lanczos(config.lanczos_max_iter, config.lanczos_tol, config.lanczos_reorth,
+                                       _compute(config))
     theta_opt = prob.extract(prob.present_after(n_steps))
     if nd == 3:
         theta_opt = _unpad_boundary_symmetric(theta_opt, pad_left=pad_left)
@@ -325,8 +355,9 @@ def _svd_and_truncate_site(theta, site: int, config: DMRGConfig, sweep_right: bo
         n = len(labels)
         left, right = list(labels[: n // 2]), list(labels[n // 2:])
     bond = f"v{site}_{site + 1}"
-    A, s, B, s_full = truncated_svd_symmetric(theta, left, right, bond, config.max_bond_dim, config.svd_trunc_err,
-                                              absorb="right" if sweep_right else "left")
+    with _phase("svd"):
+        A, s, B, s_full = truncated_svd_symmetric(theta, left, right, bond, config.max_bond_dim,
+                                                  config.svd_trunc_err, absorb="right" if sweep_right else "left")
     n_keep = len(s)
     if len(s_full) > n_keep:
         tot = float(torch.sum(s_full ** 2))

@@ -136,8 +136,10 @@ __global__ __launch_bounds__(GEMM_THREADS) void grouped_gemm_kernel(
     double *__restrict__ C, int accumulate) {
   constexpr int NT = (COMPUTE == 2) ? 2 : 1;  // hi/lo planes for split
   using T = typename std::conditional<COMPUTE == 0, double, float>::type;
-  __shared__ T As[NT][GEMM_BK * APITCH];
-  __shared__ T Bs[NT][GEMM_BK * BPITCH];
+  // Two LDS stages: tile t is read from stage t&1 while tile t+1 is written
+  // to the other stage, so each K-tile needs a single barrier.
+  __shared__ T As[2][NT][GEMM_BK * APITCH];
+  __shared__ T Bs[2][NT][GEMM_BK * BPITCH];
 
   const GemmTask t = tasks[blockIdx.x];
   const GemmOut o = outs[t.out];
@@ -159,25 +161,85 @@ __global__ __launch_bounds__(GEMM_THREADS) void grouped_gemm_kernel(
       facc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
     }
 
-  // Flattened iteration over (pair, k-tile).
+  auto stage_store = [&](int buf, const Stage &st) {
+    if (COMPUTE == 2)
+      store_tiles_split<ALAY>((float *)As[buf][0], (float *)As[buf][NT - 1], (float *)Bs[buf][0],
+                              (float *)Bs[buf][NT - 1], st, tid);
+    else
+      store_tiles<ALAY, T>(As[buf][0], Bs[buf][0], st, tid);
+  };
+
+  auto compute_tile = [&](int buf) {
+#pragma unroll
+    for (int kk = 0; kk < GEMM_BK; kk += 4) {
+      const int kr = kk + lk;
+      if (COMPUTE == 0) {
+        const double *as = (const double *)As[buf][0];
+        const double *bs = (const double *)Bs[buf][0];
+        double a0 = as[kr * APITCH + wm + lrow];
+        double a1 = as[kr * APITCH + wm + 16 + lrow];
+        double b0 = bs[kr * BPITCH + wn + lrow];
+        double b1 = bs[kr * BPITCH + wn + 16 + lrow];
+        dacc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, dacc[0][0], 0, 0, 0);
+        dacc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, dacc[0][1], 0, 0, 0);
+        dacc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, dacc[1][0], 0, 0, 0);
+        dacc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, dacc[1][1], 0, 0, 0);
+      } else if (COMPUTE == 1) {
+        const float *as = (const float *)As[buf][0];
+        const float *bs = (const float *)Bs[buf][0];
+        float a0 = as[kr * APITCH + wm + lrow];
+        float a1 = as[kr * APITCH + wm + 16 + lrow];
+        float b0 = bs[kr * BPITCH + wn + lrow];
+        float b1 = bs[kr * BPITCH + wn + 16 + lrow];
+        facc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, facc[0][0], 0, 0, 0);
+        facc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, facc[0][1], 0, 0, 0);
+        facc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, facc[1][0], 0, 0, 0);
+        facc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, facc[1][1], 0, 0, 0);
+      } else {
+        const float *ah = (const float *)As[buf][0];
+        const float *al = (const float *)As[buf][NT - 1];
+        const float *bh = (const float *)Bs[buf][0];
+        const float *bl = (const float *)Bs[buf][NT - 1];
+        float a0h = ah[kr * APITCH + wm + lrow], a0l = al[kr * APITCH + wm + lrow];
+        float a1h = ah[kr * APITCH + wm + 16 + lrow], a1l = al[kr * APITCH + wm + 16 + lrow];
+        float b0h = bh[kr * BPITCH + wn + lrow], b0l = bl[kr * BPITCH + wn + lrow];
+        float b1h = bh[kr * BPITCH + wn + 16 + lrow], b1l = bl[kr * BPITCH + wn + 16 + lrow];
+#define TNX_SPLIT3(ACC, AH, AL, BH, BL)                             \
+  ACC = __builtin_amdgcn_mfma_f32_16x16x4f32(AL, BH, ACC, 0, 0, 0); \
+  ACC = __builtin_amdgcn_mfma_f32_16x16x4f32(AH, BL, ACC, 0, 0, 0); \
+  ACC = __builtin_amdgcn_mfma_f32_16x16x4f32(AH, BH, ACC, 0, 0, 0);
+        TNX_SPLIT3(facc[0][0], a0h, a0l, b0h, b0l)
+        TNX_SPLIT3(facc[0][1], a0h, a0l, b1h, b1l)
+        TNX_SPLIT3(facc[1][0], a1h, a1l, b0h, b0l)
+        TNX_SPLIT3(facc[1][1], a1h, a1l, b1h, b1l)
+#undef TNX_SPLIT3
+      }
+    }
+    if (COMPUTE == 2) {
+      // flush the FP32 chunk sums into FP64 accumulators
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) dacc[i][j][r] += (double)facc[i][j][r];
+          facc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+        }
+    }
+  };
+
+  // Flattened iteration over (pair, k-tile) with a one-tile register prefetch.
   int pi = o.pair_begin;
-  int k0 = 0;
-  if (pi >= o.pair_end) {
-    // no contributing pair: the tile is zero (or untouched when accumulating)
-  } else {
+  if (pi < o.pair_end) {
+    int k0 = 0;
     GemmPair p = pairs[pi];
     Stage st;
     load_a<ALAY>(A, p, o.M, t.m0, k0, tid, st.a);
     load_b(B, p, o.N, t.n0, k0, tid, st.b);
+    stage_store(0, st);
+    __syncthreads();
+    int buf = 0;
     while (true) {
-      __syncthreads();
-      if (COMPUTE == 2)
-        store_tiles_split<ALAY>((float *)As[0], (float *)As[NT - 1], (float *)Bs[0],
-                                      (float *)Bs[NT - 1], st, tid);
-      else
-        store_tiles<ALAY, T>(As[0], Bs[0], st, tid);
-      __syncthreads();
-      // prefetch next tile into registers
       int nk0 = k0 + GEMM_BK;
       int npi = pi;
       GemmPair np = p;
@@ -191,63 +253,11 @@ __global__ __launch_bounds__(GEMM_THREADS) void grouped_gemm_kernel(
         load_a<ALAY>(A, np, o.M, t.m0, nk0, tid, st.a);
         load_b(B, np, o.N, t.n0, nk0, tid, st.b);
       }
-#pragma unroll
-      for (int kk = 0; kk < GEMM_BK; kk += 4) {
-        const int kr = kk + lk;
-        if (COMPUTE == 0) {
-          const double *as = (const double *)As[0];
-          const double *bs = (const double *)Bs[0];
-          double a0 = as[kr * APITCH + wm + lrow];
-          double a1 = as[kr * APITCH + wm + 16 + lrow];
-          double b0 = bs[kr * BPITCH + wn + lrow];
-          double b1 = bs[kr * BPITCH + wn + 16 + lrow];
-          dacc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, dacc[0][0], 0, 0, 0);
-          dacc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, dacc[0][1], 0, 0, 0);
-          dacc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, dacc[1][0], 0, 0, 0);
-          dacc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, dacc[1][1], 0, 0, 0);
-        } else if (COMPUTE == 1) {
-          const float *as = (const float *)As[0];
-          const float *bs = (const float *)Bs[0];
-          float a0 = as[kr * APITCH + wm + lrow];
-          float a1 = as[kr * APITCH + wm + 16 + lrow];
-          float b0 = bs[kr * BPITCH + wn + lrow];
-          float b1 = bs[kr * BPITCH + wn + 16 + lrow];
-          facc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, facc[0][0], 0, 0, 0);
-          facc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, facc[0][1], 0, 0, 0);
-          facc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, facc[1][0], 0, 0, 0);
-          facc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, facc[1][1], 0, 0, 0);
-        } else {
-          const float *ah = (const float *)As[0];
-          const float *al = (const float *)As[NT - 1];
-          const float *bh = (const float *)Bs[0];
-          const float *bl = (const float *)Bs[NT - 1];
-          float a0h = ah[kr * APITCH + wm + lrow], a0l = al[kr * APITCH + wm + lrow];
-          float a1h = ah[kr * APITCH + wm + 16 + lrow], a1l = al[kr * APITCH + wm + 16 + lrow];
-          float b0h = bh[kr * BPITCH + wn + lrow], b0l = bl[kr * BPITCH + wn + lrow];
-          float b1h = bh[kr * BPITCH + wn + 16 + lrow], b1l = bl[kr * BPITCH + wn + 16 + lrow];
-#define TNX_SPLIT3(ACC, AH, AL, BH, BL)                                     \
-  ACC = __builtin_amdgcn_mfma_f32_16x16x4f32(AL, BH, ACC, 0, 0, 0);         \
-  ACC = __builtin_amdgcn_mfma_f32_16x16x4f32(AH, BL, ACC, 0, 0, 0);         \
-  ACC = __builtin_amdgcn_mfma_f32_16x16x4f32(AH, BH, ACC, 0, 0, 0);
-          TNX_SPLIT3(facc[0][0], a0h, a0l, b0h, b0l)
-          TNX_SPLIT3(facc[0][1], a0h, a0l, b1h, b1l)
-          TNX_SPLIT3(facc[1][0], a1h, a1l, b0h, b0l)
-          TNX_SPLIT3(facc[1][1], a1h, a1l, b1h, b1l)
-#undef TNX_SPLIT3
-        }
-      }
-      if (COMPUTE == 2) {
-        // flush the FP32 chunk sums into FP64 accumulators
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-          for (int j = 0; j < 2; ++j) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) dacc[i][j][r] += (double)facc[i][j][r];
-            facc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-          }
-      }
+      compute_tile(buf);
       if (!more) break;
+      stage_store(buf ^ 1, st);
+      __syncthreads();
+      buf ^= 1;
       pi = npi;
       k0 = nk0;
       p = np;

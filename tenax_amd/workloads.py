@@ -86,8 +86,8 @@ def synthetic_dmrg_mps(L: int, chi: int, seed: int = 2, width2: float = 18.0):
 
     Site i has legs (v_{i-1,i} IN, p_i IN, v_{i,i+1} OUT) (boundary sites drop
     the missing bond), physical charges [1, -1], total 2Sz = 0, blocks N(0,1)
-    from PRNGKey(seed + i): the layout of build_random_symmetric_mps with
-    bond dimension up to chi.
+    from PRNGKey(seed + i) scaled to unit norm per site: the layout of
+    build_random_symmetric_mps with bond dimension up to chi.
     """
     from tenax_amd.core.index import FlowDirection, TensorIndex
     from tenax_amd.core.symmetry import U1Symmetry
@@ -109,7 +109,8 @@ def synthetic_dmrg_mps(L: int, chi: int, seed: int = 2, width2: float = 18.0):
             idx = (TensorIndex(sym, bonds[i - 1], FlowDirection.IN, f"v{i - 1}_{i}"),
                    TensorIndex(sym, phys, FlowDirection.IN, f"p{i}"),
                    TensorIndex(sym, bonds[i], FlowDirection.OUT, f"v{i}_{i + 1}"))
-        mps.add_node(i, SymmetricTensor.random_normal(idx, PRNGKey(seed + i), target=0 if i == L - 1 else None))
+        t = SymmetricTensor.random_normal(idx, PRNGKey(seed + i), target=0 if i == L - 1 else None)
+        mps.add_node(i, t * (1.0 / float(t.norm())))
     for i in range(L - 1):
         mps.connect(i, f"v{i}_{i + 1}", i + 1, f"v{i}_{i + 1}")
     return mps
