@@ -144,12 +144,15 @@ def run_sweep(pd, mps, mpo, chi, per_block_bonds=2):
     c = L // 2 - 1
     cmp = {}
     for name, fn in (("fused", ops.two_site_update), ("per_block", pd._per_block_two_site)):
+        fn(mps[c], mps[c + 1], left[c], mpo[c], mpo[c + 1], right[c + 2], cfg)  # warm-up (plans, libraries)
+        steps = []
         torch.cuda.synchronize()
         t0 = time.time()
         for i in range(c, c + per_block_bonds):
             fn(mps[i], mps[i + 1], left[i], mpo[i], mpo[i + 1], right[i + 2], cfg)
+            steps.append(pd.LAST_STEPS["n"])
         torch.cuda.synchronize()
-        cmp[name] = time.time() - t0
+        cmp[name] = {"seconds": time.time() - t0, "lanczos_steps": steps}
     pd.TIMERS.clear()
     pd.TIMERS["enabled"] = True
     torch.cuda.synchronize()
@@ -292,7 +295,7 @@ def main():
         result["sweep_phases_s"] = phases
         result["two_site_updates_centre_s"] = {
             "bonds": 2, "fused": cmp["fused"], "per_block": cmp["per_block"],
-            "speedup_fused": cmp["per_block"] / cmp["fused"]}
+            "speedup_fused": cmp["per_block"]["seconds"] / cmp["fused"]["seconds"]}
         log(f"sweep {sw:.2f}s phases {phases} updates {cmp}")
     if not args.no_cpu:
         dt, err = cpu_baseline(prob, (Wl, Wr))

@@ -204,6 +204,8 @@ _ACTIVE_CONFIG: dict = {}
 # Optional wall-clock accounting of the sweep phases (device-synchronised at
 # phase boundaries while enabled; used by the benchmark).
 TIMERS: dict = {"enabled": False}
+# Krylov steps taken by the most recent two-site update (either path).
+LAST_STEPS: dict = {"n": 0}
 
 
 class _phase:
@@ -323,6 +325,7 @@ def _two_site_update_symmetric(site_l, site_r, left_env, mpo_l, mpo_r, right_env
     with _phase("lanczos"):
         energy, n_steps = prob.lanczos(config.lanczos_max_iter, config.lanczos_tol, config.lanczos_reorth,
                                        _compute(config))
+    LAST_STEPS["n"] = n_steps
     theta_opt = prob.extract(prob.present_after(n_steps))
     if nd == 3:
         theta_opt = _unpad_boundary_symmetric(theta_opt, pad_left=pad_left)
@@ -484,6 +487,7 @@ def _per_block_two_site(site_l, site_r, left_env, mpo_l, mpo_r, right_env, confi
             break
         basis.append(w * (1.0 / b))
     n = len(alphas)
+    LAST_STEPS["n"] = n
     if n == 1:
         e, xv = alphas[0], basis[0]
     else:

@@ -15,8 +15,23 @@ Mirrors reference src/tenax/linalg.py:34-333 and 665-825:
 Sector visit order here: first appearance in the sorted block order (the
 reference visits dict insertion order; only exact ties across sectors can
 tell the two apart).  Row/column sub-keys inside a sector are likewise in
-sorted order.  Sector matrices are assembled on the device and decomposed with
-torch.linalg.svd (rocSOLVER) in float64.
+sorted order.
+
+Decomposition.  Without ``absorb`` each sector matrix goes through
+torch.linalg.svd (rocSOLVER gesvd) in float64.  With ``absorb`` (the DMRG
+split) only one factor has to be an isometry: the other one carries the
+singular values.  The isometric side then comes from a Gram eigenproblem,
+``M M^T = U diag(s^2) U^T`` (absorb="right") or ``M^T M = V diag(s^2) V^T``
+(absorb="left"), and the carried side is the exact product ``U^T M`` or
+``M V``; s is the row (column) norm of that product.  Eigenvectors are
+orthonormal to working precision whatever the spectrum, the carried factor
+needs no division by s, and ``U U^T M`` is the optimal rank-k approximation up
+to eigenvector error eps*s_max^2/gap, which only matters for the truncation
+when two singular values straddling the cut are degenerate to that level.
+The eigenproblems of all sectors are solved in one batched torch.linalg.eigh
+call: Gram matrices are scaled by their trace and padded to a common size
+with -1 on the padded diagonal, so padded eigenpairs sit at -1 and the
+sector's own ones in [0, 1] come out exactly decoupled.
 """
 
 from __future__ import annotations
@@ -78,6 +93,64 @@ def _pack(indices, items, device):
     return SymmetricTensor._wrap(indices, keys, shapes, offs, data.contiguous())
 
 
+# rocSOLVER's batched eigh costs about the same per call for a batch of 1 or
+# of 12 (its tridiagonal reduction is a sequence of launches, ~20 us per row),
+# so all sectors go into ONE padded batch unless that batch would exceed this.
+_MAX_BATCH_BYTES = 1 << 30
+
+
+def _batches(sizes):
+    """Groups of sector indices sharing one padded eigh call, with the pad size."""
+    P = -(-max(sizes) // 32) * 32
+    if len(sizes) * P * P * 8 <= _MAX_BATCH_BYTES:
+        return [(P, list(range(len(sizes))))]
+    return [(n, [j]) for j, n in enumerate(sizes)]
+
+
+def _gram_factors(mats, absorb):
+    """Per sector matrix M: (U, s, Vh) with s absorbed on the ``absorb`` side.
+
+    absorb="right": U from eigh(M M^T) (isometry), Vh = U^T M (= diag(s) Vh).
+    absorb="left":  Vh from eigh(M^T M) (isometry), U = M V (= U diag(s)).
+    Columns come sorted by descending s, s being the norms of the carried
+    factor's rows (columns).  min(R, C) values per sector, as the SVD gives.
+    """
+    row_side = absorb == "right"
+    grams = [M @ M.T if row_side else M.T @ M for M in mats]
+    dev = mats[0].device
+    vecs = [None] * len(mats)
+    for P, js in _batches([g.shape[0] for g in grams]):
+        stack = torch.zeros((len(js), P, P), dtype=torch.float64, device=dev)
+        for b, j in enumerate(js):
+            g = grams[j]
+            n = g.shape[0]
+            scale = torch.diagonal(g).sum().clamp_min(1e-300)
+            stack[b, :n, :n] = g / scale
+            if n < P:
+                pad = torch.arange(n, P, device=dev)
+                stack[b, pad, pad] = -1.0
+        _, V = torch.linalg.eigh(stack)
+        for b, j in enumerate(js):
+            n = grams[j].shape[0]
+            vecs[j] = V[b, :n, P - n:].flip(-1)
+    out = []
+    for M, V in zip(mats, vecs):
+        k = min(M.shape)
+        V = V[:, :k]
+        if row_side:
+            B = V.T @ M
+            s = torch.linalg.vector_norm(B, dim=1)
+        else:
+            B = M @ V
+            s = torch.linalg.vector_norm(B, dim=0)
+        s, perm = torch.sort(s, descending=True, stable=True)
+        if row_side:
+            out.append((V[:, perm], s, B[perm, :]))
+        else:
+            out.append((B[:, perm], s, V[:, perm].T))
+    return out
+
+
 def truncated_svd_symmetric(tensor: SymmetricTensor, left_labels, right_labels, new_bond_label,
                             max_singular_values=None, max_truncation_err=None, normalize=False, absorb=None):
     """Returns (U, s, Vh, s_full); ``absorb`` in {None, "left", "right"} scales
@@ -91,7 +164,7 @@ def truncated_svd_symmetric(tensor: SymmetricTensor, left_labels, right_labels, 
     dev = t.data.device
     sectors = _sector_layout(t, nl)
     results = {}
-    s_list, q_list = [], []
+    s_list, q_list, mats = [], [], []
     for q, sec in sectors.items():
         R, C = sec["R"], sec["C"]
         if R == 0 or C == 0:
@@ -102,12 +175,19 @@ def truncated_svd_symmetric(tensor: SymmetricTensor, left_labels, right_labels, 
             nc = sec["cols"][rk][0]
             r0, c0 = sec["row_off"][lk], sec["col_off"][rk]
             M[r0:r0 + nr, c0:c0 + nc] = t.block(i).reshape(nr, nc)
-        U, s, Vh = torch.linalg.svd(M, full_matrices=False)
-        results[q] = (U, s, Vh)
-        s_list.append(s)
+        mats.append(M)
         q_list.append(q)
-    if not s_list:
+    if not mats:
         raise ValueError("SVD of a tensor with no blocks")
+    if absorb is None:
+        for q, M in zip(q_list, mats):
+            U, s, Vh = torch.linalg.svd(M, full_matrices=False)
+            results[q] = (U, s, Vh)
+            s_list.append(s)
+    else:
+        for q, fac in zip(q_list, _gram_factors(mats, absorb)):
+            results[q] = fac
+            s_list.append(fac[1])
     s_cat = torch.cat(s_list)
     s_sorted, order = torch.sort(s_cat, descending=True, stable=True)
     s_host = s_sorted.cpu().numpy()
@@ -144,16 +224,15 @@ def truncated_svd_symmetric(tensor: SymmetricTensor, left_labels, right_labels, 
         nq = n_per[q]
         if nq == 0:
             continue
-        U, s, Vh = results[q]
-        sq = s[:nq]
-        if normalize and float(s_sorted[:n_keep].sum()) > 0:
-            sq = sq / s_sorted[:n_keep].sum()
+        U, _, Vh = results[q]
         Uq = U[:, :nq]
         Vq = Vh[:nq, :]
-        if absorb == "left":
-            Uq = Uq * sq[None, :]
-        elif absorb == "right":
-            Vq = Vq * sq[:, None]
+        # with absorb the Gram factors already carry s on the absorbing side
+        if normalize and absorb is not None and float(s_sorted[:n_keep].sum()) > 0:
+            if absorb == "left":
+                Uq = Uq / s_sorted[:n_keep].sum()
+            else:
+                Vq = Vq / s_sorted[:n_keep].sum()
         sec = sectors[q]
         for lk, (nr, shp) in sec["rows"].items():
             r0 = sec["row_off"][lk]

@@ -86,8 +86,12 @@ def synthetic_dmrg_mps(L: int, chi: int, seed: int = 2, width2: float = 18.0):
 
     Site i has legs (v_{i-1,i} IN, p_i IN, v_{i,i+1} OUT) (boundary sites drop
     the missing bond), physical charges [1, -1], total 2Sz = 0, blocks N(0,1)
-    from PRNGKey(seed + i) scaled to unit norm per site: the layout of
-    build_random_symmetric_mps with bond dimension up to chi.
+    from PRNGKey(seed + i): the layout of build_random_symmetric_mps with bond
+    dimension up to chi.  Each site is scaled to squared norm sqrt(D_l * D_r)
+    (D = 1 for a missing bond), which makes it an isometry on average in both
+    directions (E[sum_p A_p^T A_p] = I for D_l = D_r), so environments built
+    from these sites keep O(1) entries over the whole chain instead of
+    shrinking by ~1/chi per site.
     """
     from tenax_amd.core.index import FlowDirection, TensorIndex
     from tenax_amd.core.symmetry import U1Symmetry
@@ -110,7 +114,9 @@ def synthetic_dmrg_mps(L: int, chi: int, seed: int = 2, width2: float = 18.0):
                    TensorIndex(sym, phys, FlowDirection.IN, f"p{i}"),
                    TensorIndex(sym, bonds[i], FlowDirection.OUT, f"v{i}_{i + 1}"))
         t = SymmetricTensor.random_normal(idx, PRNGKey(seed + i), target=0 if i == L - 1 else None)
-        mps.add_node(i, t * (1.0 / float(t.norm())))
+        dl = len(bonds[i - 1]) if i > 0 else 1
+        dr = len(bonds[i]) if i < L - 1 else 1
+        mps.add_node(i, t * (float(dl * dr) ** 0.25 / float(t.norm())))
     for i in range(L - 1):
         mps.connect(i, f"v{i}_{i + 1}", i + 1, f"v{i}_{i + 1}")
     return mps
