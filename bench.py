@@ -8,11 +8,17 @@ and split-FP32 variants with their error, one full 50-step device Lanczos
 solve, the dominant kernel's roofline, and the oracle's per-combination CPU
 matvec (the reference's loop structure) on the same problem.
 
+With ``--workload hotrg`` it instead runs the batched Z2 HOTRG workload: 64
+2D-Ising temperatures T = 2.0 + 0.6 k/63, chi = 48, 30 coarse-graining steps,
+all instances in lockstep (one batched eigh per step).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--sweep]
+                       [--workload dmrg|hotrg]
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
 
 import argparse
+import importlib
 import json
 import os
 import sys
@@ -195,6 +201,95 @@ def run_c1_parity():
             "ed_anchor": gold["ed_anchor_2sz0"], "sweeps": len(res.energies_per_sweep), "seconds": dt}
 
 
+def run_hotrg(args):
+    """64-temperature Z2 HOTRG at chi=48, 30 steps; free energy vs Onsager."""
+    import torch
+
+    H = importlib.import_module("tenax_amd.algorithms.hotrg")
+    from tenax_amd.algorithms.trg import compute_ising_tensor, ising_free_energy_exact
+    from tenax_amd.contraction import contractor
+
+    n, chi, steps = args.temps, args.hotrg_chi, args.hotrg_steps
+    temps = [2.0 + 0.6 * k / max(1, n - 1) for k in range(n)]
+    cfg = H.HOTRGConfig(max_bond_dim=chi, num_steps=steps)
+    H.hotrg_batched([compute_ising_tensor(1 / temps[0], symmetric=True)], H.HOTRGConfig(chi, 8))  # warm-up
+    Ts = [compute_ising_tensor(1 / T, symmetric=True) for T in temps]
+    contractor.WORK.update(enabled=True, flops=0.0, launches=0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lnz = H.hotrg_batched(Ts, cfg)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    flops = contractor.WORK["flops"]
+    contractor.WORK["enabled"] = False
+    f = [-l * T for l, T in zip(lnz, temps)]
+    ex = [ising_free_energy_exact(1 / T) for T in temps]
+    rel = [abs(a - b) / abs(b) for a, b in zip(f, ex)]
+    log(f"hotrg {n} temps chi={chi} {steps} steps: {dt:.2f}s, {flops:.3e} flops, max rel err vs Onsager "
+        f"{max(rel):.2e}")
+    # phase split on a second, timed-by-phase run of a few instances
+    H.TIMERS.clear()
+    H.TIMERS["enabled"] = True
+    H.hotrg_batched(Ts[: min(8, n)], cfg)
+    phases = {k: v for k, v in H.TIMERS.items() if k != "enabled"}
+    H.TIMERS["enabled"] = False
+    result = {
+        "metric": "batched Z2 HOTRG GFLOP/s (block contractions), 64 Ising temperatures", "value": flops / dt / 1e9,
+        "unit": "GFLOP/s", "n_gpus": 1, "steps": steps, "warmup": 1, "ms_per_step": dt / steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"2D Ising Z2 HOTRG, {n} temperatures in [2.0, 2.6], chi={chi}, {steps} steps, "
+                               "alternating directions, all instances in lockstep", "temperatures": n, "chi": chi},
+        "seconds": dt, "contraction_flops": flops, "launches": contractor.WORK["launches"],
+        "phases_s_first8": phases,
+        "free_energy": {"max_rel_err_vs_onsager": max(rel), "per_T": [[T, a, b] for T, a, b in zip(temps, f, ex)]},
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = hotrg_cpu_baseline(chi)
+    print(json.dumps(result), flush=True)
+
+
+def hotrg_cpu_baseline(chi):
+    """One chi-sized horizontal HOTRG step of the oracle (per-block NumPy
+    contractions, LAPACK SVD) vs the same step on the GPU, same input."""
+    import numpy as np
+    import torch
+
+    from oracle import core as oc
+    from oracle import hotrg as oh
+    H = importlib.import_module("tenax_amd.algorithms.hotrg")
+    from tenax_amd.algorithms.trg import compute_ising_tensor
+    from tenax_amd.contraction import contractor
+
+    T = compute_ising_tensor(1 / 2.27, symmetric=True)
+    k = 0
+    while min(int(ix.dim) for ix in T.indices) < chi:
+        T, _ = (H._hotrg_step_horizontal if k % 2 == 0 else H._hotrg_step_vertical)(T, chi)
+        k += 1
+    if k % 2:
+        T, _ = H._hotrg_step_vertical(T, chi)
+    host = T.data.cpu().numpy()
+    legs = tuple(oc.Leg(oc.Zn(2), ix.charges, int(ix.flow), ix.label) for ix in T.indices)
+    blocks = {tuple(int(q) for q in key): host[int(o):int(o) + int(np.prod(sh))].reshape(tuple(int(x) for x in sh))
+              for key, sh, o in zip(T.keys, T.shapes, T.offsets)}
+    To = oc.SymT(blocks, legs, validate=False)
+    contractor.WORK.update(enabled=True, flops=0.0, launches=0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, ln_gpu = H._hotrg_step_horizontal(T, chi)
+    torch.cuda.synchronize()
+    gpu_s = time.perf_counter() - t0
+    flops = contractor.WORK["flops"]
+    contractor.WORK["enabled"] = False
+    t0 = time.perf_counter()
+    _, ln_cpu = oh.step_horizontal(To, chi, order="chi5")
+    cpu_s = time.perf_counter() - t0
+    log(f"hotrg cpu step {cpu_s:.2f}s gpu step {gpu_s:.3f}s |dln| {abs(ln_cpu - ln_gpu):.1e}")
+    return {"value": flops / cpu_s / 1e9, "unit": "GFLOP/s", "cores": os.cpu_count(), "kind": "port",
+            "sample": f"one horizontal HOTRG step at chi={chi} (T=2.27) by the oracle (per-block NumPy "
+                      "contractions + LAPACK SVD, OpenBLAS threads = cores); value = GPU planner flops / CPU time",
+            "seconds": cpu_s, "gpu_seconds_same_step": gpu_s, "abs_diff_log_norm": abs(ln_cpu - ln_gpu)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,11 +302,18 @@ def main():
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--perblock-steps", type=int, default=2)
     ap.add_argument("--sweep", action="store_true", help="also time one full DMRG sweep on the chi state")
+    ap.add_argument("--workload", choices=("dmrg", "hotrg"), default="dmrg")
+    ap.add_argument("--temps", type=int, default=64)
+    ap.add_argument("--hotrg-chi", type=int, default=48)
+    ap.add_argument("--hotrg-steps", type=int, default=30)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "hotrg":
+        run_hotrg(args)
+        return
     import torch
 
     from tenax_amd import _lib

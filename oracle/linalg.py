@@ -25,10 +25,12 @@ def group_blocks_by_bond_charge(t, left_axes, right_axes):
 
 
 def truncated_svd_symmetric(t, left_labels, right_labels, new_bond_label,
-                            max_singular_values=None, max_truncation_err=None):
+                            max_singular_values=None, max_truncation_err=None, driver="gesdd"):
     """linalg.py:75-333. Returns (U, s, Vh, s_full, sectors).
 
     ``sectors`` lists (q, n_rows, n_cols) per SVD'd sector in visit order.
+    ``driver``: LAPACK routine (numpy's default gesdd, or gesvd), used to
+    measure how much a result depends on the SVD implementation.
     """
     lab = t.labels()
     la = [lab.index(x) for x in left_labels]
@@ -51,7 +53,12 @@ def truncated_svd_symmetric(t, left_labels, right_labels, new_bond_label,
             li, ri = lseen[lk], rseen[rk]
             r0, c0 = sum(lsz[:li]), sum(rsz[:ri])
             M[r0:r0 + lsz[li], c0:c0 + rsz[ri]] = blk.reshape(lsz[li], rsz[ri])
-        U, s, Vh = np.linalg.svd(M, full_matrices=False)
+        if driver == "gesdd":
+            U, s, Vh = np.linalg.svd(M, full_matrices=False)
+        else:
+            import scipy.linalg
+
+            U, s, Vh = scipy.linalg.svd(M, full_matrices=False, lapack_driver=driver)
         res[q] = (U, s, Vh, list(lseen), list(rseen), lsz, rsz)
         sectors.append((q, R, C))
     pairs = []

@@ -39,12 +39,17 @@ from tenax_amd.algorithms import (  # noqa: E402
     AutoMPO,
     DMRGConfig,
     DMRGResult,
+    HOTRGConfig,
     SweepOps,
     build_auto_mpo,
     build_mpo_heisenberg,
     build_random_symmetric_mps,
+    compute_ising_tensor,
     compute_mps_sector,
     dmrg,
+    hotrg,
+    hotrg_batched,
+    ising_free_energy_exact,
     validate_mps_sector,
 )
 from tenax_amd.linalg import svd, truncated_svd  # noqa: E402

@@ -39,6 +39,10 @@ _CHARS = string.ascii_lowercase + string.ascii_uppercase
 
 _state = {"compute": "f64", "grouped": True}
 
+# Algorithmic work of the pairwise block contractions executed while
+# ``enabled`` (flops = sum of 2*M*N*K over block GEMMs, from the plans).
+WORK = {"enabled": False, "flops": 0.0, "launches": 0}
+
 
 def set_compute(mode: str):
     """Block-GEMM arithmetic: "f64" (FP64 MFMA), "f32" or "split_f32"."""
@@ -125,6 +129,9 @@ def _uniform_target(t: SymmetricTensor):
 
 
 def _execute_pair(plan, A, B, out):
+    if WORK["enabled"]:
+        WORK["flops"] += plan.stats()["flops"]
+        WORK["launches"] += 1
     lib = _lib.load()
     fn = lib.tnx_contract_execute if _state["grouped"] else lib.tnx_contract_execute_perblock
     _lib.check(fn(plan.ptr, _lib.dptr(A.data), _lib.dptr(B.data), _lib.dptr(out),

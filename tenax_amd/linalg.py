@@ -15,7 +15,8 @@ Mirrors reference src/tenax/linalg.py:34-333 and 665-825:
 Sector visit order here: first appearance in the sorted block order (the
 reference visits dict insertion order; only exact ties across sectors can
 tell the two apart).  Row/column sub-keys inside a sector are likewise in
-sorted order.
+sorted order.  Blocks enter the sector matrix reshaped in their stored leg
+order, exactly as the reference's linalg.py:181 does (see _assemble).
 
 Decomposition.  Without ``absorb`` each sector matrix goes through
 torch.linalg.svd (rocSOLVER gesvd) in float64.  With ``absorb`` (the DMRG
@@ -45,11 +46,11 @@ from tenax_amd.core.index import FlowDirection, Label, TensorIndex
 from tenax_amd.core.tensor import DenseTensor, SymmetricTensor, Tensor
 
 
-def _sector_layout(t: SymmetricTensor, nl: int):
-    """Group blocks of a tensor whose first nl legs are the row legs."""
+def _sector_layout(t: SymmetricTensor, lax, rax):
+    """Group blocks by the fused flow*charge of the left legs (linalg.py:34-69)."""
     sym = t.indices[0].symmetry
-    flows = np.array([int(ix.flow) for ix in t.indices[:nl]], np.int64)
-    qs = (t.keys[:, :nl].astype(np.int64) * flows).sum(axis=1)
+    flows = np.array([int(t.indices[i].flow) for i in lax], np.int64)
+    qs = (t.keys[:, lax].astype(np.int64) * flows).sum(axis=1)
     n = sym.n_values()
     if n is not None:
         qs = np.mod(qs, n)
@@ -57,14 +58,14 @@ def _sector_layout(t: SymmetricTensor, nl: int):
     for i in range(t.n_blocks):
         q = int(qs[i])
         sec = sectors.setdefault(q, {"rows": {}, "cols": {}, "blocks": []})
-        lk = tuple(int(x) for x in t.keys[i, :nl])
-        rk = tuple(int(x) for x in t.keys[i, nl:])
-        nr = int(np.prod(t.shapes[i, :nl]))
-        nc = int(np.prod(t.shapes[i, nl:]))
+        lk = tuple(int(t.keys[i, a]) for a in lax)
+        rk = tuple(int(t.keys[i, a]) for a in rax)
+        lshape = tuple(int(t.shapes[i, a]) for a in lax)
+        rshape = tuple(int(t.shapes[i, a]) for a in rax)
         if lk not in sec["rows"]:
-            sec["rows"][lk] = (nr, tuple(int(x) for x in t.shapes[i, :nl]))
+            sec["rows"][lk] = (int(np.prod(lshape)), lshape)
         if rk not in sec["cols"]:
-            sec["cols"][rk] = (nc, tuple(int(x) for x in t.shapes[i, nl:]))
+            sec["cols"][rk] = (int(np.prod(rshape)), rshape)
         sec["blocks"].append((i, lk, rk))
     for sec in sectors.values():
         r0 = 0
@@ -95,103 +96,109 @@ def _pack(indices, items, device):
 
 # rocSOLVER's batched eigh costs about the same per call for a batch of 1 or
 # of 12 (its tridiagonal reduction is a sequence of launches, ~20 us per row),
-# so all sectors go into ONE padded batch unless that batch would exceed this.
-_MAX_BATCH_BYTES = 1 << 30
+# so matrices are padded to a common size and batched, in chunks of at most
+# this many bytes of padded input.
+_MAX_BATCH_BYTES = 4 << 30
 
 
 def _batches(sizes):
-    """Groups of sector indices sharing one padded eigh call, with the pad size."""
+    """Chunks of matrix indices sharing one padded eigh call, with the pad size."""
     P = -(-max(sizes) // 32) * 32
-    if len(sizes) * P * P * 8 <= _MAX_BATCH_BYTES:
-        return [(P, list(range(len(sizes))))]
-    return [(n, [j]) for j, n in enumerate(sizes)]
+    per = max(1, _MAX_BATCH_BYTES // (P * P * 8))
+    idx = list(range(len(sizes)))
+    return [(P, idx[i:i + per]) for i in range(0, len(idx), per)]
 
 
-def _gram_factors(mats, absorb):
-    """Per sector matrix M: (U, s, Vh) with s absorbed on the ``absorb`` side.
-
-    absorb="right": U from eigh(M M^T) (isometry), Vh = U^T M (= diag(s) Vh).
-    absorb="left":  Vh from eigh(M^T M) (isometry), U = M V (= U diag(s)).
-    Columns come sorted by descending s, s being the norms of the carried
-    factor's rows (columns).  min(R, C) values per sector, as the SVD gives.
-    """
-    row_side = absorb == "right"
-    grams = [M @ M.T if row_side else M.T @ M for M in mats]
-    dev = mats[0].device
-    vecs = [None] * len(mats)
+def _eigvecs_desc(grams, pad_value=-1.0, normalise=True):
+    """Eigenpairs of symmetric matrices, columns by descending eigenvalue, from
+    as few batched torch.linalg.eigh calls as the size cap allows.  Matrices
+    are scaled into [-1, 1] (trace for PSD Gram matrices, Frobenius norm
+    otherwise) and padded with ``pad_value`` on the diagonal, below every
+    eigenvalue of the scaled matrix, so padded pairs separate exactly.
+    Returns [(eigenvalues, eigenvectors)] in the original scale."""
+    dev = grams[0].device
+    out = [None] * len(grams)
     for P, js in _batches([g.shape[0] for g in grams]):
         stack = torch.zeros((len(js), P, P), dtype=torch.float64, device=dev)
+        scales = []
         for b, j in enumerate(js):
             g = grams[j]
             n = g.shape[0]
-            scale = torch.diagonal(g).sum().clamp_min(1e-300)
-            stack[b, :n, :n] = g / scale
+            sc = (torch.diagonal(g).sum() if normalise else torch.linalg.matrix_norm(g)).clamp_min(1e-300)
+            scales.append(sc)
+            stack[b, :n, :n] = g / sc
             if n < P:
                 pad = torch.arange(n, P, device=dev)
-                stack[b, pad, pad] = -1.0
-        _, V = torch.linalg.eigh(stack)
+                stack[b, pad, pad] = pad_value
+        w, V = torch.linalg.eigh(stack)
         for b, j in enumerate(js):
             n = grams[j].shape[0]
-            vecs[j] = V[b, :n, P - n:].flip(-1)
-    out = []
-    for M, V in zip(mats, vecs):
-        k = min(M.shape)
-        V = V[:, :k]
-        if row_side:
-            B = V.T @ M
-            s = torch.linalg.vector_norm(B, dim=1)
-        else:
-            B = M @ V
-            s = torch.linalg.vector_norm(B, dim=0)
-        s, perm = torch.sort(s, descending=True, stable=True)
-        if row_side:
-            out.append((V[:, perm], s, B[perm, :]))
-        else:
-            out.append((B[:, perm], s, V[:, perm].T))
+            out[j] = (w[b, P - n:].flip(-1) * scales[b], V[b, :n, P - n:].flip(-1))
     return out
 
 
-def truncated_svd_symmetric(tensor: SymmetricTensor, left_labels, right_labels, new_bond_label,
-                            max_singular_values=None, max_truncation_err=None, normalize=False, absorb=None):
-    """Returns (U, s, Vh, s_full); ``absorb`` in {None, "left", "right"} scales
-    U or Vh by s along the new bond (dmrg.py:874-880 done in place)."""
-    labels = tensor.labels()
-    lax = [labels.index(l) for l in left_labels]
-    rax = [labels.index(l) for l in right_labels]
-    perm = tuple(lax + rax)
-    t = tensor if perm == tuple(range(tensor.ndim)) else tensor.transpose(perm)
-    nl = len(lax)
-    dev = t.data.device
-    sectors = _sector_layout(t, nl)
-    results = {}
-    s_list, q_list, mats = [], [], []
-    for q, sec in sectors.items():
-        R, C = sec["R"], sec["C"]
-        if R == 0 or C == 0:
-            continue
-        M = torch.zeros((R, C), dtype=torch.float64, device=dev)
-        for i, lk, rk in sec["blocks"]:
-            nr = sec["rows"][lk][0]
-            nc = sec["cols"][rk][0]
-            r0, c0 = sec["row_off"][lk], sec["col_off"][rk]
-            M[r0:r0 + nr, c0:c0 + nc] = t.block(i).reshape(nr, nc)
-        mats.append(M)
-        q_list.append(q)
-    if not mats:
-        raise ValueError("SVD of a tensor with no blocks")
+def _factor_all(mats, absorb):
+    """Per sector matrix M: (U, s, Vh) with columns/rows by descending s.
+
+    absorb=None: rocSOLVER SVD.
+    "right": U from eigh(M M^T) (isometry), Vh = U^T M (= diag(s) Vh).
+    "left":  Vh from eigh(M^T M) (isometry), U = M V (= U diag(s)).
+      s = norms of the rows of U^T M (columns of M V).
+    "isometries": both factors as isometries with the SVD's pairing, from the
+      eigenvectors of the Jordan-Wielandt matrix [[0, M], [M^T, 0]], whose
+      eigenpairs are (+-s_i, (u_i, +-v_i)/sqrt(2)).  Its eigh resolves s with
+      absolute error eps*s_max, like a direct SVD (a Gram matrix squares the
+      spread and loses the vectors of s below ~1e-8 s_max).
+    min(R, C) values per sector.
+    """
     if absorb is None:
-        for q, M in zip(q_list, mats):
-            U, s, Vh = torch.linalg.svd(M, full_matrices=False)
-            results[q] = (U, s, Vh)
-            s_list.append(s)
-    else:
-        for q, fac in zip(q_list, _gram_factors(mats, absorb)):
-            results[q] = fac
-            s_list.append(fac[1])
-    s_cat = torch.cat(s_list)
-    s_sorted, order = torch.sort(s_cat, descending=True, stable=True)
-    s_host = s_sorted.cpu().numpy()
-    order = order.cpu().numpy()
+        return [torch.linalg.svd(M, full_matrices=False) for M in mats]
+    if absorb == "isometries":
+        augs = []
+        for M in mats:
+            R, C = M.shape
+            A = torch.zeros((R + C, R + C), dtype=torch.float64, device=M.device)
+            A[:R, R:] = M
+            A[R:, :R] = M.T
+            augs.append(A)
+        out = []
+        for M, (w, X) in zip(mats, _eigvecs_desc(augs, pad_value=-2.0, normalise=False)):
+            R, C = M.shape
+            k = min(R, C)
+            U = X[:R, :k]
+            V = X[R:, :k]
+            U = U / torch.linalg.vector_norm(U, dim=0).clamp_min(1e-300)
+            V = V / torch.linalg.vector_norm(V, dim=0).clamp_min(1e-300)
+            out.append((U, w[:k].clamp_min(0.0), V.T))
+        return out
+    grams = [M @ M.T for M in mats] if absorb == "right" else [M.T @ M for M in mats]
+    out = []
+    for M, (_, V) in zip(mats, _eigvecs_desc(grams)):
+        k = min(M.shape)
+        V = V[:, :k]
+        if absorb == "left":
+            B = M @ V
+            s, perm = torch.sort(torch.linalg.vector_norm(B, dim=0), descending=True, stable=True)
+            out.append((B[:, perm], s, V[:, perm].T))
+        else:
+            B = V.T @ M
+            s, perm = torch.sort(torch.linalg.vector_norm(B, dim=1), descending=True, stable=True)
+            out.append((V[:, perm], s, B[perm, :]))
+    return out
+
+
+# singular values below this fraction of the largest are treated as noise
+_NOISE = 1e-12
+
+
+def _orthonormal_columns(X):
+    Q, R = torch.linalg.qr(X)
+    d = torch.sign(torch.diagonal(R))
+    d = torch.where(d == 0, torch.ones_like(d), d)
+    return Q * d[None, :]
+
+
+def _keep_count(s_host, max_singular_values, max_truncation_err):
     n_total = len(s_host)
     n_keep = n_total
     if max_truncation_err is not None and n_total > 0:
@@ -205,34 +212,112 @@ def truncated_svd_symmetric(tensor: SymmetricTensor, left_labels, right_labels, 
                     break
     if max_singular_values is not None:
         n_keep = min(n_keep, int(max_singular_values))
-    n_keep = max(1, min(n_keep, n_total))
-    # map global position -> (sector, index)
+    return max(1, min(n_keep, n_total))
+
+
+def _assemble(t: SymmetricTensor, lax, rax):
+    """Sector matrices of t in sector visit order.
+
+    Each stored block is reshaped to (rows, cols) in its STORED leg order, as
+    the reference does (linalg.py:181: ``block.reshape(rows, cols)`` with no
+    transpose to left+right order).  When the left legs lead the tensor this
+    is the textbook matricisation; otherwise the reference factorises a
+    reshuffled sector matrix, and so does this.  U and Vh blocks are then
+    shaped by the left / right legs (linalg.py:268-309).
+    """
+    sectors = _sector_layout(t, lax, rax)
+    q_list, mats = [], []
+    for q, sec in sectors.items():
+        R, C = sec["R"], sec["C"]
+        if R == 0 or C == 0:
+            continue
+        M = torch.zeros((R, C), dtype=torch.float64, device=t.data.device)
+        for i, lk, rk in sec["blocks"]:
+            nr = sec["rows"][lk][0]
+            nc = sec["cols"][rk][0]
+            r0, c0 = sec["row_off"][lk], sec["col_off"][rk]
+            M[r0:r0 + nr, c0:c0 + nc] = t.block(i).reshape(nr, nc)
+        mats.append(M)
+        q_list.append(q)
+    if not mats:
+        raise ValueError("SVD of a tensor with no blocks")
+    return sectors, q_list, mats
+
+
+def truncated_svd_symmetric_many(tensors, left_labels, right_labels, new_bond_label, max_singular_values=None,
+                                 max_truncation_err=None, normalize=False, absorb=None):
+    """truncated_svd_symmetric over several tensors with the same labels, whose
+    sector eigenproblems share one batched eigh (e.g. many HOTRG instances).
+    Each tensor is truncated on its own (global order within the tensor)."""
+    prepared = []
+    all_mats = []
+    for t in tensors:
+        labels = t.labels()
+        lax = [labels.index(l) for l in left_labels]
+        rax = [labels.index(l) for l in right_labels]
+        sectors, q_list, mats = _assemble(t, lax, rax)
+        prepared.append((t, lax, rax, sectors, q_list, len(all_mats)))
+        all_mats.extend(mats)
+    factors = _factor_all(all_mats, absorb)
+    return [_truncate_and_pack(t, lax, rax, sectors, q_list, all_mats[j0:j0 + len(q_list)],
+                               factors[j0:j0 + len(q_list)], new_bond_label, max_singular_values,
+                               max_truncation_err, normalize, absorb)
+            for t, lax, rax, sectors, q_list, j0 in prepared]
+
+
+def truncated_svd_symmetric(tensor: SymmetricTensor, left_labels, right_labels, new_bond_label,
+                            max_singular_values=None, max_truncation_err=None, normalize=False, absorb=None):
+    """Returns (U, s, Vh, s_full).
+
+    ``absorb``: None (plain SVD), "right"/"left" (s carried by Vh / U, the DMRG
+    split of dmrg.py:874-880 done in place), or "isometries" (U and Vh both
+    isometries as from the SVD, s not absorbed; the HOTRG use, hotrg.py:137-146).
+    """
+    return truncated_svd_symmetric_many([tensor], left_labels, right_labels, new_bond_label, max_singular_values,
+                                        max_truncation_err, normalize, absorb)[0]
+
+
+def _truncate_and_pack(t, lax, rax, sectors, q_list, mats, factors, new_bond_label, max_singular_values,
+                       max_truncation_err, normalize, absorb):
+    dev = t.data.device
+    s_list = [f[1] for f in factors]
+    s_cat = torch.cat(s_list)
+    s_sorted, order = torch.sort(s_cat, descending=True, stable=True)
+    s_host = s_sorted.cpu().numpy()
+    order = order.cpu().numpy()
+    n_keep = _keep_count(s_host, max_singular_values, max_truncation_err)
     sec_of = np.concatenate([np.full(len(s), j, np.int64) for j, s in enumerate(s_list)])
     kept = order[:n_keep]
     bond_charges = np.array([q_list[sec_of[g]] for g in kept], np.int32)
-    n_per = {q: 0 for q in q_list}
-    for g in kept:
-        n_per[q_list[sec_of[g]]] += 1
+    n_per = np.bincount(sec_of[kept], minlength=len(q_list))
     s_final = s_sorted[:n_keep]
-    if normalize and float(s_final.sum()) > 0:
-        s_final = s_final / s_final.sum()
+    s_sum = float(s_final.sum())
+    if normalize and s_sum > 0:
+        s_final = s_final / s_sum
     sym = t.indices[0].symmetry
     bond_out = TensorIndex(sym, bond_charges, FlowDirection.OUT, new_bond_label)
     bond_in = TensorIndex(sym, bond_charges, FlowDirection.IN, new_bond_label)
     U_items, V_items = [], []
-    for q in q_list:
-        nq = n_per[q]
+    for j, q in enumerate(q_list):
+        nq = int(n_per[j])
         if nq == 0:
             continue
-        U, _, Vh = results[q]
+        U, _, Vh = factors[j]
         Uq = U[:, :nq]
         Vq = Vh[:nq, :]
+        if absorb == "isometries" and float(factors[j][1][nq - 1]) <= _NOISE * float(s_host[0]):
+            # kept vectors at the noise floor (e.g. zero singular values when
+            # nothing is truncated): the (u, v) split of their Jordan-Wielandt
+            # eigenvectors is undetermined, so complete both sides to
+            # orthonormal sets (QR keeps the well-resolved leading columns)
+            Uq = _orthonormal_columns(Uq)
+            Vq = _orthonormal_columns(Vq.T).T
         # with absorb the Gram factors already carry s on the absorbing side
-        if normalize and absorb is not None and float(s_sorted[:n_keep].sum()) > 0:
+        if normalize and absorb in ("left", "right") and s_sum > 0:
             if absorb == "left":
-                Uq = Uq / s_sorted[:n_keep].sum()
+                Uq = Uq / s_sum
             else:
-                Vq = Vq / s_sorted[:n_keep].sum()
+                Vq = Vq / s_sum
         sec = sectors[q]
         for lk, (nr, shp) in sec["rows"].items():
             r0 = sec["row_off"][lk]
@@ -240,8 +325,8 @@ def truncated_svd_symmetric(tensor: SymmetricTensor, left_labels, right_labels, 
         for rk, (nc, shp) in sec["cols"].items():
             c0 = sec["col_off"][rk]
             V_items.append(((q,) + rk, Vq[:, c0:c0 + nc].reshape((nq,) + shp)))
-    U_idx = tuple(t.indices[:nl]) + (bond_out,)
-    V_idx = (bond_in,) + tuple(t.indices[nl:])
+    U_idx = tuple(t.indices[a] for a in lax) + (bond_out,)
+    V_idx = (bond_in,) + tuple(t.indices[a] for a in rax)
     return _pack(U_idx, U_items, dev), s_final, _pack(V_idx, V_items, dev), s_sorted
 
 

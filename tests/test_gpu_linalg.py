@@ -98,3 +98,21 @@ def test_gram_factorisation_at_chi256(gpu):
         got = _dense_product(U, s.cpu().numpy(), Vh, absorb)
         assert np.abs(got - ref).max() < 1e-11 * smax
         assert _isometry(U, Vh, absorb) < 1e-13
+
+
+@pytest.mark.parametrize("absorb", [None, "isometries"])
+def test_non_leading_left_legs_follow_reference_reshape(gpu, theta10, absorb):
+    """Left legs not leading: each block enters its sector matrix reshaped in
+    stored leg order (linalg.py:181), like the oracle."""
+    from tenax_amd.linalg import truncated_svd_symmetric
+
+    labs = list(theta10.labels())
+    left, right = [labs[0], labs[2]], [labs[1], labs[3]]
+    Uo, so, Vo, sfo, _ = ola.truncated_svd_symmetric(theta10, left, right, "bond", 12)
+    U, s, Vh, sf = truncated_svd_symmetric(from_oracle(theta10), left, right, "bond", 12, absorb=absorb)
+    smax = float(sfo[0])
+    assert np.abs(s.cpu().numpy() - so).max() < 1e-12 * smax
+    assert np.array_equal(U.indices[-1].charges, Uo.legs[-1].charges)
+    ref = np.tensordot(Uo.todense() * so, Vo.todense(), axes=([2], [0]))
+    got = _dense_product(U, s.cpu().numpy(), Vh, None)
+    assert np.abs(got - ref).max() < 1e-11 * smax
