@@ -134,8 +134,57 @@ def _execute_pair(plan, A, B, out):
         WORK["launches"] += 1
     lib = _lib.load()
     fn = lib.tnx_contract_execute if _state["grouped"] else lib.tnx_contract_execute_perblock
-    _lib.check(fn(plan.ptr, _lib.dptr(A.data), _lib.dptr(B.data), _lib.dptr(out),
+    _lib.check(fn(plan.ptr, _lib.dptr(A.data.detach()), _lib.dptr(B.data.detach()), _lib.dptr(out),
                   _lib.COMPUTE[_state["compute"]], _lib.stream_handle()))
+
+
+def _project_onto(R: SymmetricTensor, X: SymmetricTensor) -> torch.Tensor:
+    """R's blocks laid out in X's packed layout (zeros where R has no block)."""
+    out = torch.zeros(X.data.numel(), dtype=torch.float64, device=X.data.device)
+    kr = R.key_index()
+    xi, ri = [], []
+    for k, i in X.key_index().items():
+        j = kr.get(k)
+        if j is not None:
+            n = int(np.prod(X.shapes[i]))
+            xi.append(np.arange(X.offsets[i], X.offsets[i] + n, dtype=np.int64))
+            ri.append(np.arange(R.offsets[j], R.offsets[j] + n, dtype=np.int64))
+    if xi:
+        out[torch.from_numpy(np.concatenate(xi)).to(out.device)] = \
+            R.data[torch.from_numpy(np.concatenate(ri)).to(out.device)]
+    return out
+
+
+class _PairExecFn(torch.autograd.Function):
+    """Grouped block GEMM C = A . B with its adjoint.
+
+    Layouts: A = [free_a..., cont...] (or [cont..., free_a...]), B = [cont...,
+    free_b...], C = [free_a..., free_b...].  The adjoint is two more block
+    contractions on the same engine, dA = G . B^T and dB = A^T . G (sector
+    matching by charge, no conservation filter), each laid out in its
+    operand's own block table; blocks absent from an operand get no gradient.
+    """
+
+    @staticmethod
+    def forward(ctx, a_data, b_data, plan, A, B, C, sa, sb, sc):
+        if C.data.numel():
+            _execute_pair(plan, A, B, C.data)
+        ctx.meta = (A, B, C, sa, sb, sc)
+        return C.data
+
+    @staticmethod
+    def backward(ctx, g):
+        A, B, C, sa, sb, sc = ctx.meta
+        G = C._with_data(g.contiguous())
+        ga = gb = None
+        with torch.no_grad():
+            if ctx.needs_input_grad[0]:
+                R = contract_pair(G, B, sc, sb, sa, A.indices, _lib.MODE_MULTI_REACHABLE, None)
+                ga = _project_onto(R, A)
+            if ctx.needs_input_grad[1]:
+                R = contract_pair(A, G, sa, sc, sb, B.indices, _lib.MODE_MULTI_REACHABLE, None)
+                gb = _project_onto(R, B)
+        return ga, gb, None, None, None, None, None, None, None
 
 
 def contract_pair(A: SymmetricTensor, B: SymmetricTensor, sa: str, sb: str, so: str,
@@ -180,9 +229,13 @@ def contract_pair(A: SymmetricTensor, B: SymmetricTensor, sa: str, sb: str, so: 
     plan, keys, shapes, offs, tot = plans.pairwise_plan(A, B, a_kfirst, b_kfirst, len(cont), nat_indices, mode,
                                                         target)
     out = torch.empty(tot, dtype=torch.float64, device=A.data.device)
-    if tot:
-        _execute_pair(plan, A, B, out)
     C = SymmetricTensor._wrap(nat_indices, keys, shapes, offs, out)
+    if torch.is_grad_enabled() and (A.data.requires_grad or B.data.requires_grad):
+        Ad = A._with_data(A.data.detach())
+        Bd = B._with_data(B.data.detach())
+        C = C._with_data(_PairExecFn.apply(A.data, B.data, plan, Ad, Bd, C, sa, sb, nat))
+    elif tot:
+        _execute_pair(plan, A, B, out)
     if nat != so:
         C = C.transpose(tuple(nat.index(c) for c in so))
     return C

@@ -366,19 +366,20 @@ class SymmetricTensor(Tensor):
         return _reorder(self, tuple(ix.dual() for ix in self._indices), new_keys, conj=True)
 
     def transpose(self, axes):
-        """Leg permutation through the device transpose plan (tensor.py:832-861)."""
+        """Leg permutation through the device transpose plan (tensor.py:832-861).
+
+        Differentiable: when the data carries a gradient the kernel runs
+        behind an autograd node whose adjoint is the inverse permutation."""
         axes = tuple(int(a) for a in axes)
         if axes == tuple(range(self.ndim)):
             return self
-        from tenax_amd.contraction.plans import transpose_plan
+        if torch.is_grad_enabled() and self._data.requires_grad:
+            from tenax_amd.contraction.plans import transpose_plan
 
-        plan, keys, shapes, offs, tot = transpose_plan(self, axes)
-        out = torch.empty(tot, dtype=torch.float64, device=self._data.device)
-        if self.n_blocks:
-            lib = _lib.load()
-            _lib.check(lib.tnx_transpose_execute(plan.ptr, _lib.dptr(self._data), _lib.dptr(out),
-                                                 _lib.stream_handle()))
-        return SymmetricTensor._wrap(tuple(self._indices[i] for i in axes), keys, shapes, offs, out)
+            _, keys, shapes, offs, _ = transpose_plan(self, axes)
+            out = _TransposeFn.apply(self._data, self, axes)
+            return SymmetricTensor._wrap(tuple(self._indices[i] for i in axes), keys, shapes, offs, out)
+        return _transpose_raw(self, axes)
 
     def norm(self):
         return torch.linalg.vector_norm(self._data) if self._data.numel() else torch.zeros(
@@ -416,6 +417,35 @@ class SymmetricTensor(Tensor):
     def __repr__(self):
         return (f"SymmetricTensor(ndim={self.ndim}, n_blocks={self.n_blocks}, nnz={self._data.numel()}, "
                 f"dtype={self.dtype}, labels={self.labels()})")
+
+
+def _transpose_raw(t: "SymmetricTensor", axes) -> "SymmetricTensor":
+    from tenax_amd.contraction.plans import transpose_plan
+
+    plan, keys, shapes, offs, tot = transpose_plan(t, axes)
+    out = torch.empty(tot, dtype=torch.float64, device=t._data.device)
+    if t.n_blocks:
+        lib = _lib.load()
+        _lib.check(lib.tnx_transpose_execute(plan.ptr, _lib.dptr(t._data.detach()), _lib.dptr(out),
+                                             _lib.stream_handle()))
+    return SymmetricTensor._wrap(tuple(t._indices[i] for i in axes), keys, shapes, offs, out)
+
+
+class _TransposeFn(torch.autograd.Function):
+    """Block transpose with its adjoint (the inverse permutation)."""
+
+    @staticmethod
+    def forward(ctx, data, t, axes):
+        res = _transpose_raw(t, axes)
+        ctx.meta = (res, axes)
+        return res._data
+
+    @staticmethod
+    def backward(ctx, g):
+        res, axes = ctx.meta
+        inv = tuple(int(i) for i in np.argsort(axes))
+        back = _transpose_raw(res._with_data(g.contiguous()), inv)
+        return back._data, None, None
 
 
 def _find_key(blocks, k):
