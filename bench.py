@@ -10,7 +10,10 @@ matvec (the reference's loop structure) on the same problem.
 
 With ``--workload hotrg`` it instead runs the batched Z2 HOTRG workload: 64
 2D-Ising temperatures T = 2.0 + 0.6 k/63, chi = 48, 30 coarse-graining steps,
-all instances in lockstep (one batched eigh per step).
+all instances in lockstep (one batched eigh per step).  With ``--workload ctm``
+the iPEPS CTMRG workload: Heisenberg gate, D = 4, d = 2, chi = 64, QR
+projectors, site tensor N(0, 1) from seed 3: CTM sweeps on the dense and the
+block-sparse path, the 100-sweep AD forward and its implicit backward.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--sweep]
                        [--workload dmrg|hotrg]
@@ -290,6 +293,93 @@ def hotrg_cpu_baseline(chi):
             "seconds": cpu_s, "gpu_seconds_same_step": gpu_s, "abs_diff_log_norm": abs(ln_cpu - ln_gpu)}
 
 
+def run_ctm(args):
+    import numpy as np
+    import torch
+
+    from tenax_amd.contraction import contractor
+    from tenax_amd.core.tensor import PRNGKey, normal
+
+    C = importlib.import_module("tenax_amd.algorithms.ctm")
+    D, d, chi = 4, 2, args.ctm_chi
+    A = normal(PRNGKey(3), (D, D, D, D, d))
+    A = A / np.linalg.norm(A)
+    H = C.heisenberg_gate()
+    res = {}
+    for name, sym in (("dense", False), ("block", True)):
+        At = C.ipeps_site_tensor(A, sym)
+        a = C._build_double_layer_tensor(At)
+        env = C.initialize_ctm_tensor_env(At, chi)
+        with torch.no_grad():
+            for _ in range(2):
+                env = C._ctm_tensor_sweep(env, a, chi, True, "qr")
+            contractor.WORK.update(enabled=True, flops=0.0, launches=0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            n = args.ctm_sweeps
+            for _ in range(n):
+                env = C._ctm_tensor_sweep(env, a, chi, True, "qr")
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / n
+        res[name] = {"sweep_ms": dt * 1e3}
+        if sym:
+            res[name]["sweep_flops"] = contractor.WORK["flops"] / n
+            res[name]["launches_per_sweep"] = contractor.WORK["launches"] / n
+        contractor.WORK["enabled"] = False
+        log(f"ctm {name}: {dt * 1e3:.1f} ms per sweep")
+    flops = res["block"]["sweep_flops"]
+    # AD forward (100 sweeps + gauge fix) and implicit backward on the block path
+    cfg = C.CTMConfig(chi=chi, max_iter=100, conv_tol=1e-10, projector_method="qr")
+    At = C.ipeps_site_tensor(A, True)
+    leaf = C._leaves([At])[0].detach().clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e = C.energy_loss(C._with_leaves([At], [leaf])[0], H, cfg)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    log(f"ctm AD forward {t_fwd:.2f}s E={float(e.detach()):.10f} iterations "
+        f"{C.LAST_BACKWARD.get('forward_iterations')}")
+    # the reference CTM does not reach a fixed point here (its corner legs swap
+    # roles between moves), so GMRES cannot converge: time one restart cycle
+    C.GMRES_OPTIONS.update(maxiter=args.gmres_cycles, progress=True)
+    t0 = time.perf_counter()
+    (g,) = torch.autograd.grad(e, [leaf])
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    log(f"ctm AD backward {t_bwd:.2f}s gmres products {C.LAST_BACKWARD.get('gmres_products')} "
+        f"residual {C.LAST_BACKWARD.get('relative_residual'):.2e}")
+    result = {
+        "metric": "iPEPS CTMRG sweep GFLOP/s (block path), D=4 chi=64", "value": flops / (res["block"]["sweep_ms"] * 1e-3)
+        / 1e9, "unit": "GFLOP/s", "n_gpus": 1, "steps": args.ctm_sweeps, "warmup": 2,
+        "ms_per_step": res["block"]["sweep_ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"1-site iPEPS, Heisenberg, D=4, d=2, CTMRG chi={chi}, QR projectors, A ~ N(0,1) "
+                               "seed 3 normalised", "chi": chi, "D": D},
+        "sweep": res, "ad": {"forward_s": t_fwd, "backward_s": t_bwd, "energy": float(e.detach()),
+                             "gmres_cycles": args.gmres_cycles,
+                             "backward_s_per_gmres_product": t_bwd / max(1, C.LAST_BACKWARD.get("gmres_products", 1)),
+                             "forward_iterations": C.LAST_BACKWARD.get("forward_iterations"),
+                             "gmres_products": C.LAST_BACKWARD.get("gmres_products"),
+                             "relative_residual": C.LAST_BACKWARD.get("relative_residual"),
+                             "grad_norm": float(torch.linalg.vector_norm(g))},
+    }
+    if not args.no_cpu:
+        from oracle import ctm as occ
+
+        a_np = occ.double_layer(A)
+        env_np = occ.init_env(D * D, chi)
+        env_np = occ.sweep(env_np, a_np, chi, True, "qr")
+        t0 = time.perf_counter()
+        env_np = occ.sweep(env_np, a_np, chi, True, "qr")
+        cpu = time.perf_counter() - t0
+        result["cpu_baseline"] = {"value": flops / cpu / 1e9, "unit": "GFLOP/s", "cores": os.cpu_count(),
+                                  "kind": "port", "seconds": cpu,
+                                  "sample": f"one CTM sweep at chi={chi} by the oracle (NumPy FP64 einsum / LAPACK, "
+                                            "OpenBLAS threads = cores); value = block-path flops / CPU time"}
+        log(f"ctm cpu sweep {cpu:.2f}s")
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -302,7 +392,10 @@ def main():
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--perblock-steps", type=int, default=2)
     ap.add_argument("--sweep", action="store_true", help="also time one full DMRG sweep on the chi state")
-    ap.add_argument("--workload", choices=("dmrg", "hotrg"), default="dmrg")
+    ap.add_argument("--workload", choices=("dmrg", "hotrg", "ctm"), default="dmrg")
+    ap.add_argument("--ctm-chi", type=int, default=64)
+    ap.add_argument("--ctm-sweeps", type=int, default=5)
+    ap.add_argument("--gmres-cycles", type=int, default=1)
     ap.add_argument("--temps", type=int, default=64)
     ap.add_argument("--hotrg-chi", type=int, default=48)
     ap.add_argument("--hotrg-steps", type=int, default=30)
@@ -313,6 +406,9 @@ def main():
         sys.exit(2)
     if args.workload == "hotrg":
         run_hotrg(args)
+        return
+    if args.workload == "ctm":
+        run_ctm(args)
         return
     import torch
 

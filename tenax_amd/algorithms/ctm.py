@@ -486,7 +486,7 @@ def _unflat(v, like):
     return out
 
 
-def _gmres(matvec, b, x0, tol, restart=20, maxiter=50):
+def _gmres(matvec, b, x0, tol, restart=20, maxiter=50, progress=False):
     """Restarted GMRES (Saad-Schultz, modified Gram-Schmidt Arnoldi, Givens
     rotations); stops when ||b - A x|| <= tol ||b||, at most ``maxiter``
     restart cycles of ``restart`` Krylov vectors (the interface of
@@ -535,6 +535,11 @@ def _gmres(matvec, b, x0, tol, restart=20, maxiter=50):
         y = np.linalg.solve(np.triu(H[:k_used, :k_used]), g[:k_used]) if k_used else np.zeros(0)
         for i in range(k_used):
             x = x + float(y[i]) * V[i]
+        if progress:
+            import sys
+
+            print(f"[gmres] cycle done: {n_mv} products, residual {abs(g[k_used]) / bnorm:.3e}", file=sys.stderr,
+                  flush=True)
         if abs(g[k_used]) <= tol * bnorm:
             break
     return x, n_mv
@@ -542,6 +547,10 @@ def _gmres(matvec, b, x0, tol, restart=20, maxiter=50):
 
 # statistics of the most recent backward (GMRES products, residual)
 LAST_BACKWARD: dict = {}
+# GMRES settings of the backward: Krylov size per restart and a cap on the
+# number of restart cycles (None = the reference's min(max_iter, 50)); with
+# ``progress`` a line per cycle goes to stderr.
+GMRES_OPTIONS: dict = {"restart": 20, "maxiter": None, "progress": False}
 
 
 class _CTMFixedPoint(torch.autograd.Function):
@@ -578,8 +587,10 @@ class _CTMFixedPoint(torch.autograd.Function):
             return _flat([torch.zeros_like(e) if r is None else r for r, e in zip(res, env_leaves)])
 
         b = _flat(g)
-        lam, n_mv = _gmres(lambda v: v - jt(v), b, b, config.conv_tol, restart=20,
-                           maxiter=min(config.max_iter, 50))
+        cap = GMRES_OPTIONS["maxiter"]
+        lam, n_mv = _gmres(lambda v: v - jt(v), b, b, config.conv_tol, restart=GMRES_OPTIONS["restart"],
+                           maxiter=min(config.max_iter, 50) if cap is None else cap,
+                           progress=GMRES_OPTIONS["progress"])
         resid = float(torch.linalg.vector_norm(b - (lam - jt(lam)))) / max(float(torch.linalg.vector_norm(b)), 1e-300)
         gr = state["graph"] if state["graph"] is not None else build()
         a_in, _, out_l = gr
@@ -659,7 +670,8 @@ def heisenberg_gate(device=None) -> torch.Tensor:
     """Sz Sz + (S+ S- + S- S+)/2, shape (2, 2, 2, 2)."""
     sz = torch.diag(torch.tensor([0.5, -0.5], dtype=torch.float64))
     sp = torch.tensor([[0.0, 1.0], [0.0, 0.0]], dtype=torch.float64)
-    H = torch.kron(sz, sz) + 0.5 * (torch.kron(sp, sp.T) + torch.kron(sp.T, sp))
+    sm = sp.T.contiguous()
+    H = torch.kron(sz, sz) + 0.5 * (torch.kron(sp, sm) + torch.kron(sm, sp))
     return H.reshape(2, 2, 2, 2).to(device or torch.device("cuda", torch.cuda.current_device()))
 
 
