@@ -72,6 +72,20 @@ def build_problem(L, chi, seed):
     return pd, prob, (left, mpo[c], mpo[c + 1], right, theta), mps, mpo
 
 
+def committed_traffic(kernel, wgs):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    --pmc summary (FETCH_SIZE / WRITE_SIZE, copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "dmrg_matvec_chi2048_traffic.md")
+    try:
+        for line in open(path):
+            f = line.split()
+            if len(f) == 4 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel and int(f[2]) == wgs:
+                return float(f[3])
+    except OSError:
+        pass
+    return None
+
+
 def time_events(fn, reps):
     import torch
 
@@ -393,6 +407,8 @@ def main():
     ap.add_argument("--perblock-steps", type=int, default=2)
     ap.add_argument("--sweep", action="store_true", help="also time one full DMRG sweep on the chi state")
     ap.add_argument("--workload", choices=("dmrg", "hotrg", "ctm"), default="dmrg")
+    ap.add_argument("--matvec-only", action="store_true", help="fused matvec and stage timing only (profiling)")
+    ap.add_argument("--calibrate-copy", action="store_true", help="run a 512 MiB device copy (counter calibration)")
     ap.add_argument("--ctm-chi", type=int, default=64)
     ap.add_argument("--ctm-sweeps", type=int, default=5)
     ap.add_argument("--gmres-cycles", type=int, default=1)
@@ -432,6 +448,19 @@ def main():
     ach = st_fl[dom] / (st_ms[dom] * 1e-3) / 1e12
     log(f"stages ms {st_ms}  flops {st_fl}")
 
+    if args.calibrate_copy:
+        # a plain device copy of known size (512 MiB read + 512 MiB written),
+        # the reference launch for scaling FETCH_SIZE / WRITE_SIZE counters
+        src = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    if args.matvec_only:
+        print(json.dumps({"matvec_ms": ms, "gflops": gflops, "stage_ms": st_ms, "stage_flops": st_fl}), flush=True)
+        return
+
     # reduced precision variants
     y64 = prob.matvec(prob.x, 0)
     extra = {}
@@ -459,6 +488,9 @@ def main():
     prob.x = x0
     log(f"lanczos 50 steps {lz_s:.3f}s E={e:.12f} n={n}")
 
+    traffic = None
+    if dom == 3 and args.L == 100 and args.chi == 2048:
+        traffic = committed_traffic("grouped_gemm_kernel<0,0>", 986)
     result = {
         "metric": METRIC,
         "value": gflops,
@@ -480,7 +512,9 @@ def main():
                    "block_pairs": stats["n_pairs"], "gemm_tiles": stats["n_tasks"]},
         "roofline": {"bound": "mfma", "kernel": f"grouped_gemm stage {dom} (FP64 MFMA)",
                      "achieved": ach, "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
-                     "frac": ach / PEAK_FP64_MFMA_TFLOPS, "traffic": None,
+                     "frac": ach / PEAK_FP64_MFMA_TFLOPS, "traffic": traffic,
+                     "traffic_source": "profiles/dmrg_matvec_chi2048_traffic.md (rocprofv3 --pmc FETCH_SIZE x2.0 + "
+                                       "WRITE_SIZE x1.0, copy-calibrated)" if traffic else None,
                      "stage_ms": st_ms, "stage_flops": st_fl},
         "per_block_gpu": {"ms": ms_pb, "gflops": flops / (ms_pb * 1e-3) / 1e9, "speedup_fused": ms_pb / ms},
         "reduced_precision": extra,
