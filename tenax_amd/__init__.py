@@ -53,4 +53,4 @@ from tenax_amd.algorithms import (  # noqa: E402
     validate_mps_sector,
 )
 from tenax_amd.linalg import svd, truncated_svd  # noqa: E402
-from tenax_amd.network import TensorNetwork  # noqa: E402
+from tenax_amd.network import NetworkBlueprint, TensorNetwork, parse_netfile  # noqa: E402

@@ -7,8 +7,9 @@ from tenax_amd.contraction.contractor import (
     contract_with_subscripts,
     get_compute,
     per_block_execution,
+    plan_order,
     set_compute,
 )
 
 __all__ = ["contract", "contract_with_subscripts", "blockwise_contract", "set_compute", "get_compute",
-           "compute_mode", "per_block_execution"]
+           "compute_mode", "per_block_execution", "plan_order"]

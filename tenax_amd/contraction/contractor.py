@@ -15,7 +15,8 @@ grouped-GEMM launch over every block pair.  Operands whose leg order does not
 matricize directly go through the device transpose first.
 
 Deliberate deviations (each pinned by the dense-einsum oracle in tests):
-three or more symmetric operands contract pairwise left to right instead of
+three or more symmetric operands contract pairwise (in the greedy order of
+``plan_order``, or left to right for ``optimize="left"``) instead of
 returning an empty tensor (reference quirk, contractor.py:375-403), and a
 failing block product raises instead of being skipped (contractor.py:458-459).
 """
@@ -271,23 +272,60 @@ def _contract_symmetric(tensors, subscripts, output_indices, optimize="auto"):
         if u is not None:
             total += u
     target = total if total != 0 else None
-    # left-to-right pairwise folding; intermediates keep every leg still needed
-    cur, cs = tensors[0], subs[0]
-    for k in range(1, len(tensors)):
-        nxt, ns = tensors[k], subs[k]
-        later = set("".join(subs[k + 1:])) | set(so)
-        keep = [c for c in cs + ns if (c not in cs or c not in ns) and (c in later)]
-        # chars appearing once overall are free; chars shared with later operands stay open
-        keep = [c for c in dict.fromkeys(keep)]
+    dims = {c: ix.dim for c, ix in c2ix.items()}
+    pairs = plan_order(subs, dims) if optimize != "left" else [(0, 1)] * (len(tensors) - 1)
+    items = list(zip(tensors, subs))
+    for i, j in pairs:
+        (a, sa), (b, sb) = items[i], items[j]
+        rest = [x[1] for k, x in enumerate(items) if k not in (i, j)]
+        later = set("".join(rest)) | set(so)
+        keep = list(dict.fromkeys(c for c in sa + sb if (c not in sa or c not in sb) and c in later))
         step_out = "".join(c for c in so if c in keep) + "".join(c for c in keep if c not in so)
-        last = k == len(tensors) - 1
-        tgt = target if last else _partial_target(cur, nxt)
-        cur = contract_pair(cur, nxt, cs, ns, step_out, tuple(c2ix[c] for c in step_out),
+        last = len(items) == 2
+        tgt = target if last else _partial_target(a, b)
+        # an open leg keeps the flow of the operand it sits on (a label's
+        # first occurrence in the network may be on a tensor not yet folded in)
+        legs = dict(zip(sb, b.indices))
+        legs.update(zip(sa, a.indices))
+        if last:
+            legs.update(zip(so, output_indices))
+        cur = contract_pair(a, b, sa, sb, step_out, tuple(legs[c] for c in step_out),
                             _lib.MODE_PAIRWISE_FILTERED, tgt)
-        cs = step_out
+        items = [x for k, x in enumerate(items) if k not in (i, j)]
+        items.insert(min(i, j), (cur, step_out))
+    cur, cs = items[0]
     if cs != so:
         cur = cur.transpose(tuple(cs.index(c) for c in so))
     return cur
+
+
+def plan_order(subs: Sequence[str], dims: dict) -> list[tuple[int, int]]:
+    """Greedy pairwise contraction order (the role of opt_einsum's path search,
+    contractor.py:150-166): repeatedly contract the connected pair whose dense
+    cost (product of the dimensions of all legs involved) is smallest, ties
+    to the earliest pair; disconnected networks fall back to the first two
+    operands.  Returns positions into the shrinking operand list (the result
+    of a step takes the smaller of the two positions)."""
+    cur = [set(s) for s in subs]
+    out = []
+    while len(cur) > 1:
+        best = None
+        for i in range(len(cur)):
+            for j in range(i + 1, len(cur)):
+                if not (cur[i] & cur[j]):
+                    continue
+                cost = 1
+                for c in cur[i] | cur[j]:
+                    cost *= dims.get(c, 1)
+                if best is None or cost < best[0]:
+                    best = (cost, i, j)
+        i, j = (0, 1) if best is None else best[1:]
+        others = set().union(*(cur[k] for k in range(len(cur)) if k not in (i, j))) if len(cur) > 2 else set()
+        merged = (cur[i] ^ cur[j]) | ((cur[i] & cur[j]) & others)
+        cur = [x for k, x in enumerate(cur) if k not in (i, j)]
+        cur.insert(i, merged)
+        out.append((i, j))
+    return out
 
 
 def _partial_target(a, b):
