@@ -74,7 +74,8 @@ def build_problem(L, chi, seed):
 
 def committed_traffic(kernel, wgs):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3
-    --pmc summary (FETCH_SIZE / WRITE_SIZE, copy-calibrated), or None."""
+    --pmc summary (FETCH_SIZE / WRITE_SIZE, copy-calibrated), or None when the
+    summary holds no line for this kernel and launch size."""
     path = os.path.join(ROOT, "profiles", "dmrg_matvec_chi2048_traffic.md")
     try:
         for line in open(path):
@@ -84,6 +85,17 @@ def committed_traffic(kernel, wgs):
     except OSError:
         pass
     return None
+
+
+TILE_SHAPES = {0: (64, 64), 1: (128, 64), 2: (64, 128), 3: (128, 128)}
+
+
+def stage_kernel(prob, stage):
+    """Kernel name (as rocprofv3 prints it) and workgroup count of a matvec GEMM stage."""
+    t = prob.plan.tiling(stage)
+    bm, bn = TILE_SHAPES[t["shape"]]
+    lay = 1 if stage == 1 else 0  # stage 1 reads L as K x M, stage 3 reads Y as M x K
+    return f"grouped_gemm_kernel<{lay},0,{bm},{bn}>", t["n_tasks"], t
 
 
 def time_events(fn, reps):
@@ -458,7 +470,8 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     if args.matvec_only:
-        print(json.dumps({"matvec_ms": ms, "gflops": gflops, "stage_ms": st_ms, "stage_flops": st_fl}), flush=True)
+        print(json.dumps({"matvec_ms": ms, "gflops": gflops, "stage_ms": st_ms, "stage_flops": st_fl,
+                          "kernels": {st: stage_kernel(prob, st)[:2] for st in (1, 3)}}), flush=True)
         return
 
     # reduced precision variants
@@ -488,9 +501,9 @@ def main():
     prob.x = x0
     log(f"lanczos 50 steps {lz_s:.3f}s E={e:.12f} n={n}")
 
-    traffic = None
-    if dom == 3 and args.L == 100 and args.chi == 2048:
-        traffic = committed_traffic("grouped_gemm_kernel<0,0>", 986)
+    kname, kwgs, _ = stage_kernel(prob, dom)
+    traffic = committed_traffic(kname, kwgs) if (args.L == 100 and args.chi == 2048) else None
+    tiling = {f"stage{st}": stage_kernel(prob, st)[2] for st in (1, 3)}
     result = {
         "metric": METRIC,
         "value": gflops,
@@ -510,7 +523,8 @@ def main():
                    "L": args.L, "chi": args.chi, "theta_blocks": int(len(prob.keys)),
                    "theta_elems": int(prob.total), "flops_per_matvec": flops,
                    "block_pairs": stats["n_pairs"], "gemm_tiles": stats["n_tasks"]},
-        "roofline": {"bound": "mfma", "kernel": f"grouped_gemm stage {dom} (FP64 MFMA)",
+        "tiling": tiling,
+        "roofline": {"bound": "mfma", "kernel": f"{kname} x {kwgs} (matvec stage {dom}, FP64 MFMA)",
                      "achieved": ach, "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
                      "frac": ach / PEAK_FP64_MFMA_TFLOPS, "traffic": traffic,
                      "traffic_source": "profiles/dmrg_matvec_chi2048_traffic.md (rocprofv3 --pmc FETCH_SIZE x2.0 + "

@@ -106,6 +106,21 @@ int tnx_plan_stats(const tnx_plan *plan, double *flops, double *bytes,
                    int32_t *n_pairs, int32_t *n_tasks);
 int tnx_plan_free(tnx_plan *plan);
 
+/* Tiling of one grouped-GEMM launch of a plan (no reference counterpart: the
+ * reference dispatches one einsum per block pair).  stage: 0 for a pairwise
+ * plan; 1 (L.theta) or 3 (.R) for a matvec / environment plan.
+ *   shape  0 = 64x64, 1 = 128x64, 2 = 64x128, 3 = 128x128 output tiles per workgroup
+ *   kchunk K tiles (of 16) per task: 0 = whole K, < 0 = chosen from the work per
+ *          workgroup slot; outputs split in K are summed by a second launch in a
+ *          fixed order (deterministic)
+ *   xcd    1 = place the tiles of one output (or tile row) on one XCD, bins balanced
+ * The result stays deterministic for a given tiling; different tilings differ
+ * by rounding.  tnx_plan_get_tiling returns the active tiling and task count. */
+int tnx_plan_set_tiling(tnx_plan *plan, int32_t stage, int32_t shape, int32_t kchunk,
+                        int32_t xcd, void *stream);
+int tnx_plan_get_tiling(const tnx_plan *plan, int32_t stage, int32_t *shape, int32_t *kchunk,
+                        int32_t *xcd, int32_t *n_tasks);
+
 /* Grouped execution: every block pair of the plan in ONE launch (output tiles
  * of uneven blocks, K-segments of all contributing pairs accumulated in a
  * fixed order, no atomics).  Replaces the per-combination opt_einsum loop of

@@ -79,6 +79,8 @@ SIGNATURES = {
     "tnx_plan_output_copy": (C.c_int, [_p, _PI32, _PI32, _PI64]),
     "tnx_plan_stats": (C.c_int, [_p, _PD, _PD, _PI32, _PI32]),
     "tnx_plan_free": (C.c_int, [_p]),
+    "tnx_plan_set_tiling": (C.c_int, [_p, _i32, _i32, _i32, _i32, _p]),
+    "tnx_plan_get_tiling": (C.c_int, [_p, _i32, _PI32, _PI32, _PI32, _PI32]),
     "tnx_contract_execute": (C.c_int, [_p, _p, _p, _p, _i32, _p]),
     "tnx_contract_execute_perblock": (C.c_int, [_p, _p, _p, _p, _i32, _p]),
     "tnx_plan_transpose": (C.c_int, [C.POINTER(Blocks), C.POINTER(Leg), _PI32, _p, C.POINTER(_p)]),
@@ -182,11 +184,12 @@ class BlockTable:
 class Plan:
     """Owning wrapper of a tnx_plan*."""
 
-    __slots__ = ("ptr", "_lib", "__weakref__")
+    __slots__ = ("ptr", "_lib", "tuned", "__weakref__")
 
     def __init__(self, ptr):
         self.ptr = ptr
         self._lib = _lib
+        self.tuned = False
 
     def __del__(self):
         if self.ptr and self._lib is not None:
@@ -206,6 +209,17 @@ class Plan:
         if n:
             check(_lib.tnx_plan_output_copy(self.ptr, i32p(keys), i32p(shapes), i64p(offs)))
         return keys, shapes, offs, tot.value
+
+    def set_tiling(self, stage: int, shape: int, kchunk: int = 0, xcd: int = 0):
+        """Tiling of one grouped-GEMM stage (0 pairwise; 1 or 3 fused): shape 0-3 =
+        64x64, 128x64, 64x128, 128x128; kchunk K tiles per task (0 whole, <0 auto);
+        xcd 1 = XCD-aware task order."""
+        check(_lib.tnx_plan_set_tiling(self.ptr, stage, shape, kchunk, xcd, stream_handle()))
+
+    def tiling(self, stage: int):
+        sh, kc, xc, nt = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        check(_lib.tnx_plan_get_tiling(self.ptr, stage, C.byref(sh), C.byref(kc), C.byref(xc), C.byref(nt)))
+        return {"shape": sh.value, "kchunk": kc.value, "xcd": xc.value, "n_tasks": nt.value}
 
     def stats(self):
         f, b = C.c_double(), C.c_double()

@@ -261,6 +261,31 @@ class TwoSiteProblem:
             di = torch.from_numpy(_ranges(offs, shapes, self.theta_pos)).to(dev)
             x[di] = theta4.data
         self.x = x
+        if not self.plan.tuned:
+            self._tune()
+            self.plan.tuned = True
+
+    def _tune(self):
+        """Pick the tiling of both grouped-GEMM stages on this problem's buffers."""
+        from tenax_amd.contraction import autotune
+
+        f = (_lib.C.c_double * 4)()
+        lib = _lib.load()
+        _lib.check(lib.tnx_fused_stage_flops(self.plan.ptr, f))
+        if not autotune.ENABLED or max(f[1], f[3]) < autotune.MIN_FLOPS:
+            return
+        L, Wl, Wr, R = self.envs
+        y = torch.empty_like(self.x)
+
+        def stage(st):
+            return lambda: _lib.check(lib.tnx_matvec_execute_stage(
+                self.plan.ptr, st, _lib.dptr(L.data), _lib.dptr(Wl.data), _lib.dptr(Wr.data), _lib.dptr(R.data),
+                _lib.dptr(self.x), _lib.dptr(y), 0, _lib.stream_handle()))
+
+        for st in range(4):  # intermediates in place for the stage-alone runs
+            stage(st)()
+        autotune.tune(self.plan, 1, stage(1), f[1], "matvec")
+        autotune.tune(self.plan, 3, stage(3), f[3], "matvec")
 
     def present_after(self, n_steps: int) -> np.ndarray:
         present = np.zeros(len(self.keys), bool)

@@ -840,7 +840,7 @@ static void ensure_perblock_fused(FusedPlan &P, hipStream_t s) {
       P.pb2_first[k] = (k == ho[o].pair_begin);
       P.pb2_off[k] = (int32_t)tasks.size();
       for (int m0 = 0; m0 < g.M; m0 += GEMM_BM)
-        for (int n0 = 0; n0 < g.N; n0 += GEMM_BN) tasks.push_back(GemmTask{k, m0, n0, 0});
+        for (int n0 = 0; n0 < g.N; n0 += GEMM_BN) tasks.push_back(whole_tile(k, m0, n0));
       P.pb2_cnt[k] = (int32_t)tasks.size() - P.pb2_off[k];
     }
   }
@@ -1183,6 +1183,12 @@ static void lanczos_run(const FusedPlan &P, const double *L, const double *Wl, c
   }
   *energy = out.energy;
   *nsteps = out.nsteps;
+}
+
+GemmProgram *fused_program(tnx_plan_s *plan, int stage) {
+  TNX_REQUIRE(stage == 1 || stage == 3, TNX_ERR_ARG, "fused plans have GEMM stages 1 and 3");
+  FusedPlan &P = static_cast<FusedPlan &>(*plan);
+  return stage == 1 ? &P.s1 : &P.s2;
 }
 
 }  // namespace tnx

@@ -1,6 +1,12 @@
 // Grouped block GEMM for gfx950: uneven output blocks, each the sum of the
-// K-segments of all block pairs that feed it, one 64x64 output tile per
-// workgroup, LDS-staged operand tiles, MFMA 16x16x4 (FP64 or FP32).
+// K-segments of all block pairs that feed it, in one launch; LDS-staged
+// operand tiles, MFMA 16x16x4 (FP64 or FP32).
+//
+// Tiling (chosen per program): 64x64, 128x64, 64x128 or 128x128 output tiles
+// per 256-thread workgroup; optionally split K (a task covers a range of the
+// output's K tiles and writes a partial tile, a second launch adds the
+// partials in slot order), and an XCD-aware task order (tiles sharing operand
+// panels dispatched to one XCD's L2).  Built by set_tiling.
 //
 // Replaces the per-combination einsum of the reference
 // (src/tenax/contraction/contractor.py:444-476, src/tenax/algorithms/dmrg.py:1097-1109):
@@ -9,103 +15,140 @@
 // pairs in a fixed order (deterministic, no atomics).
 #include "tnx_internal.h"
 
+#include <type_traits>
+
 namespace tnx {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-// LDS tile row pitch (elements): +1 breaks the power-of-two stride between k rows.
-constexpr int APITCH = GEMM_BM + 1;
-constexpr int BPITCH = GEMM_BN + 1;
-
-// Per-thread staging of one A tile (64 x 16) and one B tile (16 x 64): 4 values each.
+// Per-thread staging of one A tile (BM x BK) and one B tile (BK x BN):
+// 4*PA and 4*PB elements per thread.
+template <int BM, int BN>
 struct Stage {
-  double a[4];
-  double b[4];
+  static constexpr int PA = BM / 64;
+  static constexpr int PB = BN / 64;
+  double a[PA][4];
+  double b[PB][4];
 };
 
-template <int ALAY>
-__device__ __forceinline__ void load_a(const double *__restrict__ A, const GemmPair &p, int M, int m0,
-                                       int k0, int tid, double (&r)[4]) {
-  if (ALAY == LAY_MK) {
-    const int m = m0 + (tid >> 2);
-    const int kq = k0 + ((tid & 3) << 2);
-    const double *src = A + p.a_off + (int64_t)m * p.lda;
+// Operand loads go through a buffer resource spanning the one block: an
+// out-of-range element gets the offset `nbytes` and the hardware returns 0.
+// The loads carry no branches or selects, so the compiler keeps them in
+// flight across the MFMAs of the previous tile.  (Blocks are < 2^31 bytes,
+// checked when the program is finalized.)
+constexpr int32_t BUF_DWORD3 = 0x00020000;  // raw dword access, gfx9 family
+
+__device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t r, int32_t off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+
+// A operand: MK = a[m*lda+k] (4 threads per row of 16 k, 4 consecutive k
+// each), KM = a[k*lda+m] (one element per thread per k row).
+template <int ALAY, int BM>
+__device__ __forceinline__ void load_a(const double *__restrict__ A, const GemmPair &p, int M, int m0, int k0,
+                                       int tid, double (&r)[BM / 64][4]) {
+  const int32_t nb = (int32_t)(8 * (ALAY == LAY_MK ? (int64_t)(M - 1) * p.lda + p.K : (int64_t)(p.K - 1) * p.lda + M));
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void *)(A + p.a_off), (short)0, nb, BUF_DWORD3);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) r[i] = (m < M && kq + i < p.K) ? src[kq + i] : 0.0;
-  } else {
-    const int k = k0 + (tid >> 4);
-    const int mq = m0 + ((tid & 15) << 2);
-    const double *src = A + p.a_off + (int64_t)k * p.lda;
+  for (int q = 0; q < BM / 64; ++q) {
+    // the run's start offset is computed unconditionally; each element then
+    // only selects between it (+8i) and the out-of-range offset
+    if (ALAY == LAY_MK) {
+      const int m = m0 + (tid >> 2) + 64 * q;
+      const int kq = k0 + ((tid & 3) << 2);
+      const int run = 8 * (m * p.lda + kq);
+      const int lim = m < M ? p.K - kq : 0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) r[i] = (k < p.K && mq + i < M) ? src[mq + i] : 0.0;
+      for (int i = 0; i < 4; ++i) r[q][i] = buf_ld(rs, i < lim ? run + 8 * i : nb);
+    } else {
+      // one element per thread per k row, consecutive lanes on consecutive m
+      // (coalesced 512-byte rows, conflict-free LDS stores)
+      constexpr int RPP = 256 / BM;  // k rows per pass
+      const int m = m0 + tid % BM;
+      const int kb = k0 + tid / BM;
+      const bool min = m < M;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int k = kb + RPP * (4 * q + i);
+        r[q][i] = buf_ld(rs, (min && k < p.K) ? 8 * (k * p.lda + m) : nb);
+      }
+    }
   }
 }
 
 // B operand is always K x N row-major (KN): b[k*ldb + n].
+template <int BN>
 __device__ __forceinline__ void load_b(const double *__restrict__ B, const GemmPair &p, int N, int n0, int k0,
-                                       int tid, double (&r)[4]) {
-  const int k = k0 + (tid >> 4);
-  const int nq = n0 + ((tid & 15) << 2);
-  const double *src = B + p.b_off + (int64_t)k * p.ldb;
+                                       int tid, double (&r)[BN / 64][4]) {
+  constexpr int RPP = 256 / BN;  // k rows per pass; one element per thread per row
+  const int32_t nb = (int32_t)(8 * ((int64_t)(p.K - 1) * p.ldb + N));
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void *)(B + p.b_off), (short)0, nb, BUF_DWORD3);
+  const int n = n0 + tid % BN;
+  const int kb = k0 + tid / BN;
+  const bool nin = n < N;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) r[i] = (k < p.K && nq + i < N) ? src[nq + i] : 0.0;
+  for (int q = 0; q < BN / 64; ++q)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int k = kb + RPP * (4 * q + i);
+      r[q][i] = buf_ld(rs, (nin && k < p.K) ? 8 * (k * p.ldb + n) : nb);
+    }
 }
 
-// Store staged registers into LDS tiles laid out [k][m] and [k][n].
-template <int ALAY, class T>
-__device__ __forceinline__ void store_tiles(T *As, T *Bs, const Stage &st, int tid) {
-  if (ALAY == LAY_MK) {
-    const int m = tid >> 2, kq = (tid & 3) << 2;
+// LDS positions ([k][m] and [k][n], pitch BM+1 / BN+1) of a thread's staged values.
+template <int ALAY, int BM>
+__device__ __forceinline__ int a_lds(int tid, int q, int i) {
+  constexpr int AP = BM + 1;
+  if (ALAY == LAY_MK) return (((tid & 3) << 2) + i) * AP + (tid >> 2) + 64 * q;
+  constexpr int RPP = 256 / BM;
+  return (tid / BM + RPP * (4 * q + i)) * AP + tid % BM;
+}
+template <int BN>
+__device__ __forceinline__ int b_lds(int tid, int q, int i) {
+  constexpr int BP = BN + 1;
+  constexpr int RPP = 256 / BN;
+  return (tid / BN + RPP * (4 * q + i)) * BP + tid % BN;
+}
+
+template <int ALAY, int BM, int BN, class T>
+__device__ __forceinline__ void store_tiles(T *As, T *Bs, const Stage<BM, BN> &st, int tid) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) As[(kq + i) * APITCH + m] = (T)st.a[i];
-  } else {
-    const int k = tid >> 4, mq = (tid & 15) << 2;
+  for (int q = 0; q < BM / 64; ++q)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) As[k * APITCH + mq + i] = (T)st.a[i];
-  }
-  {
-    const int k = tid >> 4, nq = (tid & 15) << 2;
+    for (int i = 0; i < 4; ++i)
+      As[a_lds<ALAY, BM>(tid, q, i)] = (T)st.a[q][i];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) Bs[k * BPITCH + nq + i] = (T)st.b[i];
-  }
+  for (int q = 0; q < BN / 64; ++q)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      Bs[b_lds<BN>(tid, q, i)] = (T)st.b[q][i];
 }
 
 // Split-FP32 staging: hi = fp32(x), lo = fp32(x - hi).
-template <int ALAY>
+template <int ALAY, int BM, int BN>
 __device__ __forceinline__ void store_tiles_split(float *Ah, float *Al, float *Bh, float *Bl,
-                                                  const Stage &st, int tid) {
-  float ah[4], al[4], bh[4], bl[4];
+                                                  const Stage<BM, BN> &st, int tid) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    ah[i] = (float)st.a[i];
-    al[i] = (float)(st.a[i] - (double)ah[i]);
-    bh[i] = (float)st.b[i];
-    bl[i] = (float)(st.b[i] - (double)bh[i]);
-  }
-  if (ALAY == LAY_MK) {
-    const int m = tid >> 2, kq = (tid & 3) << 2;
+  for (int q = 0; q < BM / 64; ++q)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      Ah[(kq + i) * APITCH + m] = ah[i];
-      Al[(kq + i) * APITCH + m] = al[i];
+      const double x = st.a[q][i];
+      const float h = (float)x;
+      const int at = a_lds<ALAY, BM>(tid, q, i);
+      Ah[at] = h;
+      Al[at] = (float)(x - (double)h);
     }
-  } else {
-    const int k = tid >> 4, mq = (tid & 15) << 2;
+#pragma unroll
+  for (int q = 0; q < BN / 64; ++q)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      Ah[k * APITCH + mq + i] = ah[i];
-      Al[k * APITCH + mq + i] = al[i];
+      const double x = st.b[q][i];
+      const float h = (float)x;
+      const int at = b_lds<BN>(tid, q, i);
+      Bh[at] = h;
+      Bl[at] = (float)(x - (double)h);
     }
-  }
-  {
-    const int k = tid >> 4, nq = (tid & 15) << 2;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      Bh[k * BPITCH + nq + i] = bh[i];
-      Bl[k * BPITCH + nq + i] = bl[i];
-    }
-  }
 }
 
 // Write one 16x16 MFMA accumulator.  Operand lanes are row/col + 16*k for both
@@ -128,146 +171,227 @@ __device__ __forceinline__ void store_acc(double *__restrict__ C, const GemmOut 
   }
 }
 
-// COMPUTE: 0 = FP64 MFMA, 1 = FP32 MFMA, 2 = split FP32 (3 products, FP64 across K chunks)
-template <int ALAY, int COMPUTE>
-__global__ __launch_bounds__(GEMM_THREADS) void grouped_gemm_kernel(
+// Partial tile into a workspace slot (BM x BN row-major, every element written).
+template <bool F64LAYOUT, int BN, class V>
+__device__ __forceinline__ void store_partial(double *__restrict__ W, int row0, int col0, int lane, const V &acc) {
+  const int col = col0 + (lane & 15);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int row = row0 + (F64LAYOUT ? ((lane >> 4) + 4 * r) : (((lane >> 4) << 2) + r));
+    W[row * BN + col] = (double)acc[r];
+  }
+}
+
+// COMPUTE: 0 = FP64 MFMA, 1 = FP32 MFMA, 2 = split FP32 (3 products, FP64 across K chunks).
+// BM x BN workgroup tile, 4 waves in 2 x 2, each wave (BM/2) x (BN/2) as
+// (BM/32) x (BN/32) MFMA 16x16 blocks.
+// At least two waves per SIMD (the 128-wide tiles held to 256 registers),
+// except split FP32 at 128x128, whose hi/lo operands would then spill.
+template <int ALAY, int COMPUTE, int BM, int BN>
+__global__ __launch_bounds__(GEMM_THREADS) __attribute__((amdgpu_waves_per_eu(
+    (COMPUTE == 2 && BM * BN > 8192) ? 1 : 2))) void grouped_gemm_kernel(
     const GemmTask *__restrict__ tasks, const GemmOut *__restrict__ outs,
     const GemmPair *__restrict__ pairs, const double *__restrict__ A, const double *__restrict__ B,
-    double *__restrict__ C, int accumulate) {
+    double *__restrict__ C, double *__restrict__ ws, int accumulate) {
   constexpr int NT = (COMPUTE == 2) ? 2 : 1;  // hi/lo planes for split
+  constexpr int AP = BM + 1, BP = BN + 1;
+  constexpr int TI = BM / 32, TJ = BN / 32;
   using T = typename std::conditional<COMPUTE == 0, double, float>::type;
   // Two LDS stages: tile t is read from stage t&1 while tile t+1 is written
   // to the other stage, so each K-tile needs a single barrier.
-  __shared__ T As[2][NT][GEMM_BK * APITCH];
-  __shared__ T Bs[2][NT][GEMM_BK * BPITCH];
+  __shared__ T As[2][NT][GEMM_BK * AP];
+  __shared__ T Bs[2][NT][GEMM_BK * BP];
 
   const GemmTask t = tasks[blockIdx.x];
+  if (t.out < 0) return;  // padding entry (uniform over the workgroup)
   const GemmOut o = outs[t.out];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wm = (wave >> 1) * 32;
-  const int wn = (wave & 1) * 32;
+  const int wm = (wave >> 1) * (BM / 2);
+  const int wn = (wave & 1) * (BN / 2);
+  // 16x16 MFMA blocks of this wave that touch the output block (edge tiles
+  // of uneven blocks have fewer): the others are skipped, which frees the
+  // matrix pipe for the other waves on the SIMD.  Wave-uniform.
+  const int na = __builtin_amdgcn_readfirstlane(min(TI, max(0, (o.M - t.m0 - wm + 15) / 16)));
+  const int nb = __builtin_amdgcn_readfirstlane(min(BN / 32, max(0, (o.N - t.n0 - wn + 15) / 16)));
   const int lrow = lane & 15;
   const int lk = lane >> 4;
 
-  d4 dacc[2][2];
-  f4 facc[2][2];
+  d4 dacc[TI][TJ];
+  f4 facc[TI][TJ];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < TI; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < TJ; ++j) {
       dacc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
       facc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
     }
 
-  auto stage_store = [&](int buf, const Stage &st) {
+  auto stage_store = [&](int buf, const Stage<BM, BN> &st) {
     if (COMPUTE == 2)
-      store_tiles_split<ALAY>((float *)As[buf][0], (float *)As[buf][NT - 1], (float *)Bs[buf][0],
-                              (float *)Bs[buf][NT - 1], st, tid);
+      store_tiles_split<ALAY, BM, BN>((float *)As[buf][0], (float *)As[buf][NT - 1], (float *)Bs[buf][0],
+                                      (float *)Bs[buf][NT - 1], st, tid);
     else
-      store_tiles<ALAY, T>(As[buf][0], Bs[buf][0], st, tid);
+      store_tiles<ALAY, BM, BN, T>(As[buf][0], Bs[buf][0], st, tid);
   };
 
-  auto compute_tile = [&](int buf) {
+  // One BK-deep tile from LDS stage `buf`: the operands of k-step kk+1 are
+  // read while the MFMAs of k-step kk run.
+  // GUARD: skip the MFMA blocks outside the output block (edge tiles only).
+  auto compute_tile = [&](int buf, auto guard) {
+    constexpr bool GUARD = decltype(guard)::value;
+    if (COMPUTE != 2) {
+      const T *as = As[buf][0];
+      const T *bs = Bs[buf][0];
+      T a[2][TI], b[2][TJ];
+#pragma unroll
+      for (int i = 0; i < TI; ++i) a[0][i] = as[lk * AP + wm + 16 * i + lrow];
+#pragma unroll
+      for (int j = 0; j < TJ; ++j) b[0][j] = bs[lk * BP + wn + 16 * j + lrow];
+#pragma unroll
+      for (int kk = 0; kk < GEMM_BK / 4; ++kk) {
+        const int cur = kk & 1;
+        if (kk + 1 < GEMM_BK / 4) {
+          const int kr = 4 * (kk + 1) + lk;
+#pragma unroll
+          for (int i = 0; i < TI; ++i) a[cur ^ 1][i] = as[kr * AP + wm + 16 * i + lrow];
+#pragma unroll
+          for (int j = 0; j < TJ; ++j) b[cur ^ 1][j] = bs[kr * BP + wn + 16 * j + lrow];
+        }
+        // keep the reads of step kk+1 ahead of the MFMAs of step kk
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < TI; ++i)
+#pragma unroll
+          for (int j = 0; j < TJ; ++j) {
+            if (GUARD && (i >= na || j >= nb)) continue;
+            if (COMPUTE == 0)
+              dacc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[cur][i], b[cur][j], dacc[i][j], 0, 0, 0);
+            else
+              facc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[cur][i], b[cur][j], facc[i][j], 0, 0, 0);
+          }
+      }
+      return;
+    }
 #pragma unroll
     for (int kk = 0; kk < GEMM_BK; kk += 4) {
       const int kr = kk + lk;
-      if (COMPUTE == 0) {
-        const double *as = (const double *)As[buf][0];
-        const double *bs = (const double *)Bs[buf][0];
-        double a0 = as[kr * APITCH + wm + lrow];
-        double a1 = as[kr * APITCH + wm + 16 + lrow];
-        double b0 = bs[kr * BPITCH + wn + lrow];
-        double b1 = bs[kr * BPITCH + wn + 16 + lrow];
-        dacc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, dacc[0][0], 0, 0, 0);
-        dacc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, dacc[0][1], 0, 0, 0);
-        dacc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, dacc[1][0], 0, 0, 0);
-        dacc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, dacc[1][1], 0, 0, 0);
-      } else if (COMPUTE == 1) {
-        const float *as = (const float *)As[buf][0];
-        const float *bs = (const float *)Bs[buf][0];
-        float a0 = as[kr * APITCH + wm + lrow];
-        float a1 = as[kr * APITCH + wm + 16 + lrow];
-        float b0 = bs[kr * BPITCH + wn + lrow];
-        float b1 = bs[kr * BPITCH + wn + 16 + lrow];
-        facc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, facc[0][0], 0, 0, 0);
-        facc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, facc[0][1], 0, 0, 0);
-        facc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, facc[1][0], 0, 0, 0);
-        facc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, facc[1][1], 0, 0, 0);
-      } else {
-        const float *ah = (const float *)As[buf][0];
-        const float *al = (const float *)As[buf][NT - 1];
-        const float *bh = (const float *)Bs[buf][0];
-        const float *bl = (const float *)Bs[buf][NT - 1];
-        float a0h = ah[kr * APITCH + wm + lrow], a0l = al[kr * APITCH + wm + lrow];
-        float a1h = ah[kr * APITCH + wm + 16 + lrow], a1l = al[kr * APITCH + wm + 16 + lrow];
-        float b0h = bh[kr * BPITCH + wn + lrow], b0l = bl[kr * BPITCH + wn + lrow];
-        float b1h = bh[kr * BPITCH + wn + 16 + lrow], b1l = bl[kr * BPITCH + wn + 16 + lrow];
-#define TNX_SPLIT3(ACC, AH, AL, BH, BL)                             \
-  ACC = __builtin_amdgcn_mfma_f32_16x16x4f32(AL, BH, ACC, 0, 0, 0); \
-  ACC = __builtin_amdgcn_mfma_f32_16x16x4f32(AH, BL, ACC, 0, 0, 0); \
-  ACC = __builtin_amdgcn_mfma_f32_16x16x4f32(AH, BH, ACC, 0, 0, 0);
-        TNX_SPLIT3(facc[0][0], a0h, a0l, b0h, b0l)
-        TNX_SPLIT3(facc[0][1], a0h, a0l, b1h, b1l)
-        TNX_SPLIT3(facc[1][0], a1h, a1l, b0h, b0l)
-        TNX_SPLIT3(facc[1][1], a1h, a1l, b1h, b1l)
-#undef TNX_SPLIT3
+      const float *ah = (const float *)As[buf][0];
+      const float *al = (const float *)As[buf][NT - 1];
+      const float *bh = (const float *)Bs[buf][0];
+      const float *bl = (const float *)Bs[buf][NT - 1];
+      float aH[TI], aL[TI], bH[TJ], bL[TJ];
+#pragma unroll
+      for (int i = 0; i < TI; ++i) {
+        aH[i] = ah[kr * AP + wm + 16 * i + lrow];
+        aL[i] = al[kr * AP + wm + 16 * i + lrow];
       }
-    }
-    if (COMPUTE == 2) {
-      // flush the FP32 chunk sums into FP64 accumulators
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < TJ; ++j) {
+        bH[j] = bh[kr * BP + wn + 16 * j + lrow];
+        bL[j] = bl[kr * BP + wn + 16 * j + lrow];
+      }
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+      for (int i = 0; i < TI; ++i)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) dacc[i][j][r] += (double)facc[i][j][r];
-          facc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < TJ; ++j) {
+          if (GUARD && (i >= na || j >= nb)) continue;
+          facc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(aL[i], bH[j], facc[i][j], 0, 0, 0);
+          facc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(aH[i], bL[j], facc[i][j], 0, 0, 0);
+          facc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(aH[i], bH[j], facc[i][j], 0, 0, 0);
         }
     }
+    // flush the FP32 chunk sums into FP64 accumulators
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+      for (int j = 0; j < TJ; ++j) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) dacc[i][j][r] += (double)facc[i][j][r];
+        facc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+      }
   };
 
-  // Flattened iteration over (pair, k-tile) with a one-tile register prefetch.
+  // Locate the task's first K tile (skip kt0 tiles over the output's pairs)
+  // and count the tiles it covers.
   int pi = o.pair_begin;
+  int skip = t.kt0;
+  GemmPair p{};
+  while (pi < o.pair_end) {
+    p = pairs[pi];
+    const int nkt = (p.K + GEMM_BK - 1) / GEMM_BK;
+    if (skip < nkt) break;
+    skip -= nkt;
+    ++pi;
+  }
+  int n = 0;
   if (pi < o.pair_end) {
-    int k0 = 0;
-    GemmPair p = pairs[pi];
-    Stage st;
-    load_a<ALAY>(A, p, o.M, t.m0, k0, tid, st.a);
-    load_b(B, p, o.N, t.n0, k0, tid, st.b);
-    stage_store(0, st);
-    __syncthreads();
-    int buf = 0;
-    while (true) {
-      int nk0 = k0 + GEMM_BK;
-      int npi = pi;
-      GemmPair np = p;
-      if (nk0 >= p.K) {
-        npi = pi + 1;
-        nk0 = 0;
-        if (npi < o.pair_end) np = pairs[npi];
-      }
-      const bool more = npi < o.pair_end;
-      if (more) {
-        load_a<ALAY>(A, np, o.M, t.m0, nk0, tid, st.a);
-        load_b(B, np, o.N, t.n0, nk0, tid, st.b);
-      }
-      compute_tile(buf);
-      if (!more) break;
-      stage_store(buf ^ 1, st);
-      __syncthreads();
-      buf ^= 1;
-      pi = npi;
-      k0 = nk0;
-      p = np;
+    int64_t avail = (p.K + GEMM_BK - 1) / GEMM_BK - skip;
+    for (int q = pi + 1; q < o.pair_end && avail < (int64_t)(t.kt1 - t.kt0); ++q)
+      avail += (pairs[q].K + GEMM_BK - 1) / GEMM_BK;
+    n = (int)(avail < (int64_t)(t.kt1 - t.kt0) ? avail : (int64_t)(t.kt1 - t.kt0));
+  }
+
+  // Load cursor over the flattened (pair, k-tile) sequence.
+  int lk0 = skip * GEMM_BK;
+  auto advance = [&]() {
+    lk0 += GEMM_BK;
+    if (lk0 >= p.K) {
+      lk0 = 0;
+      if (pi + 1 < o.pair_end) p = pairs[pi + 1];
+      ++pi;
     }
+  };
+  auto load = [&](Stage<BM, BN> &st) {
+    load_a<ALAY, BM>(A, p, o.M, t.m0, lk0, tid, st.a);
+    load_b<BN>(B, p, o.N, t.n0, lk0, tid, st.b);
+    advance();
+  };
+
+  // Pipeline: tile i is computed from LDS stage i&1 while tile i+1 waits in
+  // registers and tile i+2 is in flight from memory (two register stages).
+  if (n > 0) {
+    Stage<BM, BN> r0, r1;
+    load(r0);
+    load(r1);
+    stage_store(0, r0);
+    __syncthreads();
+    // step(i, X, Y): X holds tile i+1 (to be stored), Y receives tile i+2.
+    // The prefetch is issued unconditionally (past the last tile it re-reads
+    // the last pair's first K tile and is never stored), so the number of
+    // loads in flight is the same on every path.
+    auto step = [&](int i, Stage<BM, BN> &X, Stage<BM, BN> &Y) {
+      load(Y);
+      compute_tile(i & 1, std::true_type{});
+      if (i + 1 < n) {
+        stage_store((i + 1) & 1, X);
+        __syncthreads();
+      }
+    };
+    for (int i = 0; i < n; i += 2) {
+      step(i, r1, r0);
+      if (i + 1 < n) step(i + 1, r0, r1);
+    }
+  }
+  if (t.slot >= 0) {
+    double *W = ws + (int64_t)t.slot * (BM * BN);
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+      for (int j = 0; j < TJ; ++j) {
+        if (COMPUTE == 1)
+          store_partial<false, BN>(W, wm + i * 16, wn + j * 16, lane, facc[i][j]);
+        else
+          store_partial<COMPUTE == 0, BN>(W, wm + i * 16, wn + j * 16, lane, dacc[i][j]);
+      }
+    return;
   }
   const bool acc_flag = accumulate != 0;
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < TI; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < TJ; ++j) {
       const int r0 = t.m0 + wm + i * 16;
       const int c0 = t.n0 + wn + j * 16;
       if (COMPUTE == 1)
@@ -277,51 +401,98 @@ __global__ __launch_bounds__(GEMM_THREADS) void grouped_gemm_kernel(
     }
 }
 
-template <int ALAY>
-static void launch_lay(int compute, dim3 grid, hipStream_t s, const GemmTask *tasks, const GemmOut *outs,
-                       const GemmPair *pairs, const double *A, const double *B, double *C, int acc) {
+// Sum the partial tiles of each split tile in slot order (deterministic) into C.
+template <int BM, int BN>
+__global__ __launch_bounds__(256) void split_reduce_kernel(const SplitTile *__restrict__ red,
+                                                           const GemmOut *__restrict__ outs,
+                                                           const double *__restrict__ ws, double *__restrict__ C,
+                                                           int accumulate) {
+  const SplitTile r = red[blockIdx.x];
+  const GemmOut o = outs[r.out];
+  for (int e = threadIdx.x; e < BM * BN; e += 256) {
+    const int row = r.m0 + e / BN, col = r.n0 + e % BN;
+    if (row >= o.M || col >= o.N) continue;
+    double v = 0.0;
+    for (int q = 0; q < r.nsplit; ++q) v += ws[(int64_t)(r.slot0 + q) * (BM * BN) + e];
+    double *dst = C + o.c_off + (int64_t)row * o.ldc + col;
+    *dst = accumulate ? *dst + v : v;
+  }
+}
+
+template <int ALAY, int BM, int BN>
+static void launch_tile(int compute, dim3 grid, hipStream_t s, const GemmTask *tasks, const GemmOut *outs,
+                        const GemmPair *pairs, const double *A, const double *B, double *C, double *ws, int acc) {
   switch (compute) {
     case TNX_COMPUTE_F64:
-      hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, 0>), grid, dim3(GEMM_THREADS), 0, s, tasks, outs, pairs, A, B,
-                         C, acc);
+      hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, 0, BM, BN>), grid, dim3(GEMM_THREADS), 0, s, tasks, outs, pairs,
+                         A, B, C, ws, acc);
       break;
     case TNX_COMPUTE_F32:
-      hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, 1>), grid, dim3(GEMM_THREADS), 0, s, tasks, outs, pairs, A, B,
-                         C, acc);
+      hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, 1, BM, BN>), grid, dim3(GEMM_THREADS), 0, s, tasks, outs, pairs,
+                         A, B, C, ws, acc);
       break;
     case TNX_COMPUTE_SPLIT_F32:
-      hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, 2>), grid, dim3(GEMM_THREADS), 0, s, tasks, outs, pairs, A, B,
-                         C, acc);
+      hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, 2, BM, BN>), grid, dim3(GEMM_THREADS), 0, s, tasks, outs, pairs,
+                         A, B, C, ws, acc);
       break;
     default:
       throw Error{TNX_ERR_DTYPE, "unknown compute mode " + std::to_string(compute)};
   }
 }
 
-static void launch_any(int a_lay, int b_lay, int compute, int n_tasks, hipStream_t s, const GemmTask *tasks,
-                       const GemmOut *outs, const GemmPair *pairs, const double *A, const double *B, double *C,
-                       int acc) {
+template <int ALAY>
+static void launch_shape(int shape, int compute, dim3 grid, hipStream_t s, const GemmTask *tasks,
+                         const GemmOut *outs, const GemmPair *pairs, const double *A, const double *B, double *C,
+                         double *ws, int acc) {
+  switch (shape) {
+    case 0: launch_tile<ALAY, 64, 64>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc); break;
+    case 1: launch_tile<ALAY, 128, 64>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc); break;
+    case 2: launch_tile<ALAY, 64, 128>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc); break;
+    case 3: launch_tile<ALAY, 128, 128>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc); break;
+    default: throw Error{TNX_ERR_ARG, "unknown tile shape " + std::to_string(shape)};
+  }
+}
+
+static void launch_any(int a_lay, int b_lay, int shape, int compute, int n_tasks, hipStream_t s,
+                       const GemmTask *tasks, const GemmOut *outs, const GemmPair *pairs, const double *A,
+                       const double *B, double *C, double *ws, int acc) {
   if (n_tasks <= 0) return;
   TNX_REQUIRE(b_lay == LAY_KN, TNX_ERR_ARG, "B operands must be K x N row-major");
   dim3 grid(n_tasks);
   if (a_lay == LAY_MK)
-    launch_lay<LAY_MK>(compute, grid, s, tasks, outs, pairs, A, B, C, acc);
+    launch_shape<LAY_MK>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc);
   else
-    launch_lay<LAY_KM>(compute, grid, s, tasks, outs, pairs, A, B, C, acc);
+    launch_shape<LAY_KM>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc);
   TNX_CHECK_HIP(hipGetLastError());
 }
 
 void launch_grouped_gemm(const GemmProgram &prog, const double *A, const double *B, double *C, int compute,
                          bool accumulate, hipStream_t s) {
-  launch_any(prog.a_lay, prog.b_lay, compute, prog.n_tasks, s, prog.tasks.p, prog.outs.p, prog.pairs.p, A, B,
-             C, accumulate ? 1 : 0);
+  const int acc = accumulate ? 1 : 0;
+  if (!prog.tiled) {
+    launch_any(prog.a_lay, prog.b_lay, 0, compute, prog.n_tasks, s, prog.tasks.p, prog.outs.p, prog.pairs.p, A, B,
+               C, nullptr, acc);
+    return;
+  }
+  launch_any(prog.a_lay, prog.b_lay, prog.cfg.shape, compute, prog.n_vtasks, s, prog.vtasks.p, prog.outs.p,
+             prog.pairs.p, A, B, C, prog.ws.p, acc);
+  if (prog.n_red > 0) {
+    const dim3 g(prog.n_red), b(256);
+    switch (prog.cfg.shape) {
+      case 0: hipLaunchKernelGGL((split_reduce_kernel<64, 64>), g, b, 0, s, prog.red.p, prog.outs.p, prog.ws.p, C, acc); break;
+      case 1: hipLaunchKernelGGL((split_reduce_kernel<128, 64>), g, b, 0, s, prog.red.p, prog.outs.p, prog.ws.p, C, acc); break;
+      case 2: hipLaunchKernelGGL((split_reduce_kernel<64, 128>), g, b, 0, s, prog.red.p, prog.outs.p, prog.ws.p, C, acc); break;
+      default: hipLaunchKernelGGL((split_reduce_kernel<128, 128>), g, b, 0, s, prog.red.p, prog.outs.p, prog.ws.p, C, acc); break;
+    }
+    TNX_CHECK_HIP(hipGetLastError());
+  }
 }
 
-// Per-block path helper: launch a prepared single-pair program.
+// Per-block path helper: launch a prepared single-pair program (64x64 whole tiles).
 void launch_prepared(int a_lay, int b_lay, int compute, int n_tasks, hipStream_t s, const GemmTask *tasks,
                      const GemmOut *outs, const GemmPair *pairs, const double *A, const double *B, double *C,
                      bool accumulate) {
-  launch_any(a_lay, b_lay, compute, n_tasks, s, tasks, outs, pairs, A, B, C, accumulate ? 1 : 0);
+  launch_any(a_lay, b_lay, 0, compute, n_tasks, s, tasks, outs, pairs, A, B, C, nullptr, accumulate ? 1 : 0);
 }
 
 }  // namespace tnx

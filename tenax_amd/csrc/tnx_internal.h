@@ -111,11 +111,38 @@ struct GemmOut {
   int32_t pad;
 };
 
-// One workgroup's output tile.
+// One workgroup's output tile, or a K-range of it (split K).  The K range
+// [kt0, kt1) counts BK-wide K tiles over the concatenation of the output's
+// pairs.  slot < 0: the tile is written to C; slot >= 0: the partial tile goes
+// to workspace slot `slot` and a reduction adds the slots in order.
+// out < 0 marks an empty entry (padding of the XCD-interleaved order).
 struct GemmTask {
   int32_t out;
   int32_t m0, n0;
+  int32_t kt0, kt1;
+  int32_t slot;
+  int32_t pad0, pad1;
+};
+
+__host__ __device__ inline GemmTask whole_tile(int32_t out, int32_t m0, int32_t n0) {
+  return GemmTask{out, m0, n0, 0, 0x7fffffff, -1, 0, 0};
+}
+
+// A tile whose K range was split: slots slot0 .. slot0+nsplit-1 hold its partials.
+struct SplitTile {
+  int32_t out, m0, n0;
+  int32_t slot0, nsplit;
   int32_t pad;
+};
+
+// Tiling of a grouped GEMM launch.
+//   shape: 0 = 64x64, 1 = 128x64, 2 = 64x128, 3 = 128x128 (rows x cols per workgroup)
+//   kchunk: K tiles per task; 0 = no split, < 0 = chosen from the work per workgroup slot
+//   xcd: order the tasks so that tiles of one output run on one XCD (shared L2)
+struct TileCfg {
+  int32_t shape = 0;
+  int32_t kchunk = 0;
+  int32_t xcd = 0;
 };
 
 enum : int { LAY_MK = 0, LAY_KM = 1 };  // A: MK = a[m*lda+k], KM = a[k*lda+m]
@@ -133,11 +160,28 @@ struct GemmProgram {
   DevBuf<GemmOut> outs;
   DevBuf<GemmTask> tasks;
   int32_t n_pairs = 0, n_outs = 0, n_tasks = 0;
-  // host mirrors (used by the per-block path and for stats)
+  // host mirrors (used by the per-block path, tiling and stats)
   std::vector<GemmPair> h_pairs;
   std::vector<GemmOut> h_outs;
   double flops = 0.0;
+  // tiled variant (set_tiling): used by launch_grouped_gemm when `tiled`
+  bool tiled = false;
+  TileCfg cfg;
+  DevBuf<GemmTask> vtasks;
+  DevBuf<SplitTile> red;
+  DevBuf<double> ws;
+  int32_t n_vtasks = 0, n_red = 0;
 };
+
+// Number of XCDs workgroups are dispatched across (round robin by workgroup id).
+constexpr int N_XCD = 8;
+// Cap on the split-K workspace of one program (doubles).
+constexpr int64_t MAX_WS_DOUBLES = int64_t(32) << 20;
+
+// Build the task list of `cfg` for a finalized program (host-side, from the
+// host mirrors) and make it the launched one.  The default 64x64 unsplit
+// device-built list stays in `tasks` (the per-block path uses it).
+void set_tiling(GemmProgram &prog, const TileCfg &cfg, hipStream_t s);
 
 // Launch the grouped GEMM: C[outs] (=|+=) sum_pairs A*B.
 void launch_grouped_gemm(const GemmProgram &prog, const double *A, const double *B, double *C,
@@ -181,3 +225,8 @@ struct tnx_plan_base {
 }  // namespace tnx
 
 struct tnx_plan_s : public tnx::tnx_plan_base {};
+
+namespace tnx {
+// The grouped-GEMM program of a fused (matvec / environment) plan's stage 1 or 3.
+GemmProgram *fused_program(tnx_plan_s *plan, int stage);
+}  // namespace tnx

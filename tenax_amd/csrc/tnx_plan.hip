@@ -17,6 +17,7 @@
 #include <rocprim/rocprim.hpp>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <numeric>
 
@@ -384,7 +385,7 @@ __global__ void write_tasks_kernel(int64_t nout, const uint32_t *__restrict__ or
   const int tm = (g.M + GEMM_BM - 1) / GEMM_BM;
   int64_t t = toff[r];
   for (int a = 0; a < tm; ++a)
-    for (int b = 0; b < tn; ++b) tasks[t++] = GemmTask{o, a * GEMM_BM, b * GEMM_BN, 0};
+    for (int b = 0; b < tn; ++b) tasks[t++] = whole_tile(o, a * GEMM_BM, b * GEMM_BN);
 }
 
 void finalize_program(GemmProgram &prog, hipStream_t s) {
@@ -427,6 +428,116 @@ void finalize_program(GemmProgram &prog, hipStream_t s) {
   for (const auto &o : prog.h_outs)
     for (int k = o.pair_begin; k < o.pair_end; ++k) f += 2.0 * o.M * (double)o.N * prog.h_pairs[k].K;
   prog.flops = f;
+}
+
+// ------------------------------------------------------------------ tiling variants (host)
+static void tile_dims(int shape, int &bm, int &bn) {
+  bm = (shape & 1) ? 128 : 64;
+  bn = (shape & 2) ? 128 : 64;
+}
+
+// Resident workgroups per CU of each tile shape (LDS-bound, FP64 staging).
+static int wg_per_cu(int shape) { return shape == 0 ? 4 : shape == 3 ? 2 : 3; }
+
+void set_tiling(GemmProgram &prog, const TileCfg &cfg, hipStream_t s) {
+  TNX_REQUIRE(cfg.shape >= 0 && cfg.shape <= 3, TNX_ERR_ARG, "tile shape must be 0..3");
+  prog.cfg = cfg;
+  prog.tiled = false;
+  prog.n_vtasks = prog.n_red = 0;
+  if (cfg.shape == 0 && cfg.kchunk == 0 && cfg.xcd == 0) return;  // the device-built default list
+  int bm, bn;
+  tile_dims(cfg.shape, bm, bn);
+  const auto &ho = prog.h_outs;
+  const auto &hp = prog.h_pairs;
+  const int64_t nout = (int64_t)ho.size();
+  std::vector<int64_t> kt(nout), ntile(nout);
+  double total = 0.0;  // K tiles summed over all output tiles
+  for (int64_t o = 0; o < nout; ++o) {
+    int64_t c = 0;
+    for (int k = ho[o].pair_begin; k < ho[o].pair_end; ++k) c += (hp[k].K + GEMM_BK - 1) / GEMM_BK;
+    kt[o] = c;
+    ntile[o] = (int64_t)((ho[o].M + bm - 1) / bm) * ((ho[o].N + bn - 1) / bn);
+    total += (double)ntile[o] * (double)std::max<int64_t>(c, 1);
+  }
+  // K tiles per task: bounded by half the mean work of one resident workgroup
+  int64_t chunk = cfg.kchunk;
+  if (chunk < 0) chunk = std::max<int64_t>(8, (int64_t)std::ceil(total / (2.0 * 256 * wg_per_cu(cfg.shape))));
+  // work units (an output, or a row of tiles of a large output) with their tasks
+  struct Unit {
+    int64_t cost;
+    int32_t out, mt0, mt1;
+  };
+  std::vector<Unit> units;
+  const double big = total / 64.0;
+  for (int64_t o = 0; o < nout; ++o) {
+    const int tm = (ho[o].M + bm - 1) / bm;
+    const int64_t per_row = (int64_t)((ho[o].N + bn - 1) / bn) * std::max<int64_t>(kt[o], 1);
+    if (cfg.xcd && tm > 1 && (double)per_row * tm > big) {
+      for (int a = 0; a < tm; ++a) units.push_back(Unit{per_row, (int32_t)o, a, a + 1});
+    } else {
+      units.push_back(Unit{per_row * tm, (int32_t)o, 0, tm});
+    }
+  }
+  std::stable_sort(units.begin(), units.end(), [](const Unit &x, const Unit &y) { return x.cost > y.cost; });
+  const int nbin = cfg.xcd ? N_XCD : 1;
+  std::vector<std::vector<GemmTask>> bins(nbin);
+  std::vector<int64_t> load(nbin, 0);
+  std::vector<SplitTile> red;
+  int64_t slot = 0;
+  for (const Unit &u : units) {
+    const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[b] += u.cost;
+    const GemmOut &g = ho[u.out];
+    const int tn = (g.N + bn - 1) / bn;
+    const int64_t K = kt[u.out];
+    const int nsplit = (chunk > 0 && K > chunk) ? (int)((K + chunk - 1) / chunk) : 1;
+    for (int a = u.mt0; a < u.mt1; ++a)
+      for (int c = 0; c < tn; ++c) {
+        if (nsplit == 1) {
+          bins[b].push_back(whole_tile(u.out, a * bm, c * bn));
+          continue;
+        }
+        red.push_back(SplitTile{u.out, a * bm, c * bn, (int32_t)slot, nsplit, 0});
+        for (int q = 0; q < nsplit; ++q) {
+          const int32_t k0 = (int32_t)(K * q / nsplit), k1 = (int32_t)(K * (q + 1) / nsplit);
+          bins[b].push_back(GemmTask{u.out, a * bm, c * bn, k0, k1, (int32_t)(slot + q), 0, 0});
+        }
+        slot += nsplit;
+      }
+  }
+  TNX_REQUIRE(slot * bm * bn <= MAX_WS_DOUBLES, TNX_ERR_CAPACITY,
+              "split-K workspace exceeds its cap; use a larger kchunk");
+  // each bin longest task first (LPT); ties keep unit order (tiles of one output together)
+  auto cost = [](const GemmTask &t, const std::vector<int64_t> &kt) {
+    return std::min<int64_t>(t.kt1, std::max<int64_t>(kt[t.out], 1)) - t.kt0;
+  };
+  size_t rounds = 0;
+  for (auto &v : bins) {
+    std::stable_sort(v.begin(), v.end(),
+                     [&](const GemmTask &x, const GemmTask &y) { return cost(x, kt) > cost(y, kt); });
+    rounds = std::max(rounds, v.size());
+  }
+  std::vector<GemmTask> tasks;
+  if (nbin == 1) {
+    tasks = std::move(bins[0]);
+  } else {
+    // workgroup id w is dispatched to XCD w % N_XCD: interleave the bins
+    tasks.assign(rounds * nbin, GemmTask{-1, 0, 0, 0, 0, -1, 0, 0});
+    for (int b = 0; b < nbin; ++b)
+      for (size_t r = 0; r < bins[b].size(); ++r) tasks[r * nbin + b] = bins[b][r];
+  }
+  TNX_REQUIRE(tasks.size() < (size_t(1) << 31), TNX_ERR_CAPACITY, "too many GEMM tiles");
+  prog.vtasks.upload(tasks.data(), tasks.size(), s);
+  prog.n_vtasks = (int32_t)tasks.size();
+  prog.n_red = (int32_t)red.size();
+  if (!red.empty()) {
+    prog.red.upload(red.data(), red.size(), s);
+    prog.ws.alloc((size_t)slot * bm * bn);
+  } else {
+    prog.ws.release();
+  }
+  TNX_CHECK_HIP(hipStreamSynchronize(s));
+  prog.tiled = true;
 }
 
 // ------------------------------------------------------------------ plan objects
@@ -603,7 +714,7 @@ static void ensure_perblock(PairwisePlan &P, hipStream_t s) {
       P.pb_first[k] = (k == ho[o].pair_begin);
       P.pb_task_off[k] = (int32_t)tasks.size();
       for (int m0 = 0; m0 < g.M; m0 += GEMM_BM)
-        for (int n0 = 0; n0 < g.N; n0 += GEMM_BN) tasks.push_back(GemmTask{k, m0, n0, 0});
+        for (int n0 = 0; n0 < g.N; n0 += GEMM_BN) tasks.push_back(whole_tile(k, m0, n0));
       P.pb_task_cnt[k] = (int32_t)tasks.size() - P.pb_task_off[k];
     }
   }
@@ -804,6 +915,39 @@ int tnx_plan_stats(const tnx_plan *plan, double *flops, double *bytes, int32_t *
 
 int tnx_plan_free(tnx_plan *plan) {
   return guarded([&] { delete plan; });
+}
+
+static GemmProgram *stage_program(tnx_plan *plan, int stage) {
+  TNX_REQUIRE(plan, TNX_ERR_ARG, "null plan");
+  if (plan->kind == PlanKind::Pairwise) {
+    TNX_REQUIRE(stage == 0, TNX_ERR_ARG, "a pairwise plan has one GEMM stage (0)");
+    return &static_cast<PairwisePlan &>(*plan).prog;
+  }
+  TNX_REQUIRE(plan->kind == PlanKind::Matvec || plan->kind == PlanKind::EnvLeft, TNX_ERR_ARG,
+              "plan has no grouped-GEMM stage");
+  return fused_program(plan, stage);
+}
+
+int tnx_plan_set_tiling(tnx_plan *plan, int32_t stage, int32_t shape, int32_t kchunk, int32_t xcd, void *stream) {
+  return guarded([&] {
+    GemmProgram *g = stage_program(plan, stage);
+    TileCfg c;
+    c.shape = shape;
+    c.kchunk = kchunk;
+    c.xcd = xcd ? 1 : 0;
+    set_tiling(*g, c, (hipStream_t)stream);
+  });
+}
+
+int tnx_plan_get_tiling(const tnx_plan *plan, int32_t stage, int32_t *shape, int32_t *kchunk, int32_t *xcd,
+                        int32_t *n_tasks) {
+  return guarded([&] {
+    const GemmProgram *g = stage_program(const_cast<tnx_plan *>(plan), stage);
+    if (shape) *shape = g->tiled ? g->cfg.shape : 0;
+    if (kchunk) *kchunk = g->tiled ? g->cfg.kchunk : 0;
+    if (xcd) *xcd = g->tiled ? g->cfg.xcd : 0;
+    if (n_tasks) *n_tasks = g->tiled ? g->n_vtasks : g->n_tasks;
+  });
 }
 
 int tnx_contract_execute(const tnx_plan *plan, const double *a, const double *b, double *c, int32_t compute,

@@ -94,6 +94,41 @@ def test_perblock_path_matches_grouped(gpu, c2_pair):
     assert d <= 1e-12 * C1.data.abs().max().item()
 
 
+TILINGS = [(sh, kc, xc) for sh in (0, 1, 2, 3) for kc in (0, 1, 3, -1) for xc in (0, 1)]
+
+
+def _last_pair_plan():
+    from tenax_amd.contraction import plans
+
+    return [v for k, v in plans._cache.items() if k[0] == "pair"][-1][0]
+
+
+@pytest.mark.parametrize("mode,tol", [("f64", 1e-13), ("f32", 2e-5), ("split_f32", 2e-6)])
+def test_contract_tilings_agree(gpu, c2_pair, mode, tol):
+    """Every tiling of the grouped GEMM (tile shape, split K -- including one K
+    tile per task, which splits across pair boundaries -- and XCD order) gives
+    the oracle's result; FP64 tilings agree with each other to rounding."""
+    from tenax_amd import compute_mode, contract
+
+    oA, oB, oC = c2_pair
+    A, B = from_oracle(oA), from_oracle(oB)
+    with compute_mode(mode):
+        C0 = contract(A, B)
+        plan = _last_pair_plan()
+        for cfg in TILINGS:
+            plan.set_tiling(0, *cfg)
+            C = contract(A, B)
+            assert plan.tiling(0)["shape"] == cfg[0]
+            assert_same_structure(C, oC)
+            assert max_rel_err(C, oC) < tol, cfg
+            if mode == "f64":
+                d = (C.data - C0.data).abs().max().item()
+                assert d <= 1e-13 * C0.data.abs().max().item(), cfg
+        plan.set_tiling(0, 0, 0, 0)
+    with pytest.raises(Exception, match="tile shape"):
+        plan.set_tiling(0, 4, 0, 0)
+
+
 def test_contract_needs_transposes_and_swap(gpu):
     """Operands whose legs interleave: device transposes + swapped roles."""
     from tenax_amd import contract

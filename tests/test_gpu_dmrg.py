@@ -97,6 +97,47 @@ def test_fused_matvec_matches_oracle(gpu, state8, site):
     assert (y - y2).abs().max().item() <= 1e-12 * y.abs().max().item()
 
 
+def test_matvec_tilings_agree(gpu, state8):
+    """Both GEMM stages of the fused matvec under every tiling (shape, split K
+    down to one K tile per task, XCD order) match the default tiling."""
+    mpo, mps = state8
+    prob, _ = _problem(mpo, mps, 3)
+    y0 = prob.matvec(prob.x, 0).clone()
+    for st in (1, 3):
+        for sh in (0, 1, 2, 3):
+            for kc in (0, 1, -1):
+                for xc in (0, 1):
+                    prob.plan.set_tiling(st, sh, kc, xc)
+                    y = prob.matvec(prob.x, 0)
+                    err = (y - y0).abs().max().item()
+                    assert err <= 1e-13 * y0.abs().max().item(), (st, sh, kc, xc)
+        prob.plan.set_tiling(st, 0, 0, 0)
+    with pytest.raises(Exception, match="stages 1 and 3"):
+        prob.plan.set_tiling(2, 0, 0, 0)
+
+
+def test_autotune_picks_a_candidate(gpu, monkeypatch):
+    """The autotuner times the candidates on the plan's buffers and applies the fastest."""
+    from tenax_amd.contraction import autotune
+
+    autotune.clear()
+    monkeypatch.setattr(autotune, "MIN_FLOPS", 0.0)
+    mpo, mps = _oracle_state(8, 12, 3)
+    prob, _ = _problem(mpo, mps, 2)
+    prob._tune()  # the plan may come from the cache untuned: tune it (or reuse the choice)
+    prob._tune()
+    for st in (1, 3):
+        t = prob.plan.tiling(st)
+        assert (t["shape"], t["kchunk"], t["xcd"]) in autotune.CANDIDATES
+    assert autotune.STATS["timed"] == 2  # one timing per stage signature
+    assert autotune.STATS["reused"] >= 2  # later plans of the same work bucket reuse it
+    ref = prob.matvec(prob.x, 0)
+    prob.plan.set_tiling(1, 0, 0, 0)
+    prob.plan.set_tiling(3, 0, 0, 0)
+    y = prob.matvec(prob.x, 0)
+    assert (y - ref).abs().max().item() <= 1e-13 * ref.abs().max().item()
+
+
 @pytest.mark.parametrize("mode,tol", [("f32", 1e-5), ("split_f32", 1e-6)])
 def test_fused_matvec_reduced_precision(gpu, state8, mode, tol):
     from tenax_amd import _lib
