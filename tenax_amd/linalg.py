@@ -98,17 +98,59 @@ def _pack(indices, items, device):
 
 # rocSOLVER's batched eigh costs about the same per call for a batch of 1 or
 # of 12 (its tridiagonal reduction is a sequence of launches, ~20 us per row),
-# so matrices are padded to a common size and batched, in chunks of at most
-# this many bytes of padded input.
+# so matrices of similar size are padded to a common size and batched, in
+# chunks of at most this many bytes of padded input.  The call also blocks its
+# host thread while the GPU is mostly idle, so batches of different sizes run
+# concurrently: each from its own host thread on its own stream (default
+# priority).  One 11-sector chi=2048 DMRG split: one padded batch 38 ms, size
+# groups in parallel ~27 ms.
 _MAX_BATCH_BYTES = 4 << 30
+_N_STREAMS = 4
+_PARALLEL_MIN = 256  # below this largest size one batch is cheapest
+_pool = None
+_streams: dict = {}
 
 
 def _batches(sizes):
-    """Chunks of matrix indices sharing one padded eigh call, with the pad size."""
-    P = -(-max(sizes) // 32) * 32
-    per = max(1, _MAX_BATCH_BYTES // (P * P * 8))
-    idx = list(range(len(sizes)))
-    return [(P, idx[i:i + per]) for i in range(0, len(idx), per)]
+    """Chunks of matrix indices sharing one padded eigh call, with the pad size.
+
+    Sizes are grouped (largest first) so that a group's padded size is at
+    most 1.2x its smallest member; beyond _N_STREAMS groups the smallest are
+    merged.  Chunks respect the byte cap."""
+    order = sorted(range(len(sizes)), key=lambda j: -sizes[j])
+    groups = []
+    for j in order:
+        if groups and (len(groups) >= _N_STREAMS or sizes[groups[-1][0]] <= 1.2 * sizes[j]
+                       or max(sizes) < _PARALLEL_MIN):
+            groups[-1].append(j)
+        else:
+            groups.append([j])
+    out = []
+    for g in groups:
+        P = -(-sizes[g[0]] // 32) * 32
+        per = max(1, _MAX_BATCH_BYTES // (P * P * 8))
+        out.extend((P, g[i:i + per]) for i in range(0, len(g), per))
+    return out
+
+
+def _eigh_one(grams, P, js, pad_value, normalise, dev):
+    stack = torch.zeros((len(js), P, P), dtype=torch.float64, device=dev)
+    scales = []
+    for b, j in enumerate(js):
+        g = grams[j]
+        n = g.shape[0]
+        sc = (torch.diagonal(g).sum() if normalise else torch.linalg.matrix_norm(g)).clamp_min(1e-300)
+        scales.append(sc)
+        stack[b, :n, :n] = g / sc
+        if n < P:
+            pad = torch.arange(n, P, device=dev)
+            stack[b, pad, pad] = pad_value
+    w, V = torch.linalg.eigh(stack)
+    res = []
+    for b, j in enumerate(js):
+        n = grams[j].shape[0]
+        res.append((j, w[b, P - n:].flip(-1) * scales[b], V[b, :n, P - n:].flip(-1)))
+    return res
 
 
 def _eigvecs_desc(grams, pad_value=-1.0, normalise=True):
@@ -118,24 +160,41 @@ def _eigvecs_desc(grams, pad_value=-1.0, normalise=True):
     otherwise) and padded with ``pad_value`` on the diagonal, below every
     eigenvalue of the scaled matrix, so padded pairs separate exactly.
     Returns [(eigenvalues, eigenvectors)] in the original scale."""
+    global _pool
     dev = grams[0].device
     out = [None] * len(grams)
-    for P, js in _batches([g.shape[0] for g in grams]):
-        stack = torch.zeros((len(js), P, P), dtype=torch.float64, device=dev)
-        scales = []
-        for b, j in enumerate(js):
-            g = grams[j]
-            n = g.shape[0]
-            sc = (torch.diagonal(g).sum() if normalise else torch.linalg.matrix_norm(g)).clamp_min(1e-300)
-            scales.append(sc)
-            stack[b, :n, :n] = g / sc
-            if n < P:
-                pad = torch.arange(n, P, device=dev)
-                stack[b, pad, pad] = pad_value
-        w, V = torch.linalg.eigh(stack)
-        for b, j in enumerate(js):
-            n = grams[j].shape[0]
-            out[j] = (w[b, P - n:].flip(-1) * scales[b], V[b, :n, P - n:].flip(-1))
+    batches = _batches([g.shape[0] for g in grams])
+    if len(batches) == 1:
+        for j, w, V in _eigh_one(grams, *batches[0], pad_value, normalise, dev):
+            out[j] = (w, V)
+        return out
+    from concurrent.futures import ThreadPoolExecutor
+
+    if _pool is None:
+        _pool = ThreadPoolExecutor(max_workers=_N_STREAMS, thread_name_prefix="tnx-eigh")
+    main = torch.cuda.current_stream(dev)
+    ss = _streams.setdefault(dev.index, [torch.cuda.Stream(device=dev) for _ in range(_N_STREAMS)])
+
+    def run(k, P, js):
+        st = ss[k % _N_STREAMS]
+        with torch.cuda.device(dev), torch.cuda.stream(st):
+            res = _eigh_one(grams, P, js, pad_value, normalise, dev)
+            for _, w, V in res:
+                w.record_stream(main)
+                V.record_stream(main)
+        return res
+
+    for st in ss:
+        st.wait_stream(main)
+    for g in grams:
+        for st in ss:
+            g.record_stream(st)
+    futs = [_pool.submit(run, k, P, js) for k, (P, js) in enumerate(batches)]
+    for f in futs:
+        for j, w, V in f.result():
+            out[j] = (w, V)
+    for st in ss:
+        main.wait_stream(st)
     return out
 
 
