@@ -33,6 +33,7 @@ sys.path.insert(0, ROOT)
 METRIC = "DMRG matvec GFLOP/s & sweep s, fused vs per-block GPU at |ΔE|<1e-10"
 PEAK_FP64_MFMA_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+GEMM_ONLY = 16  # TNX_STAGE_GEMM_ONLY (include/tnx_dmrg.h)
 
 
 def log(*a):
@@ -129,6 +130,12 @@ def stage_times(prob, reps, compute=0):
     out = {}
     for st in range(4):
         out[st] = time_events(lambda: _lib.check(lib.tnx_matvec_execute_stage(*args(st))), reps)
+    # the GEMM kernel of stages 1 and 3 alone (without a split-K reduction):
+    # the launch that rocprofv3 lists as the dominant kernel
+    for st in (1, 3):
+        out[f"{st}_gemm"] = time_events(lambda: _lib.check(lib.tnx_matvec_execute_stage(*args(st | GEMM_ONLY))), reps)
+    for st in range(4):  # leave complete intermediates behind
+        _lib.check(lib.tnx_matvec_execute_stage(*args(st)))
     f4 = (C.c_double * 4)()
     _lib.check(lib.tnx_fused_stage_flops(prob.plan.ptr, f4))
     return out, list(f4)
@@ -421,6 +428,8 @@ def main():
     ap.add_argument("--workload", choices=("dmrg", "hotrg", "ctm"), default="dmrg")
     ap.add_argument("--matvec-only", action="store_true", help="fused matvec and stage timing only (profiling)")
     ap.add_argument("--calibrate-copy", action="store_true", help="run a 512 MiB device copy (counter calibration)")
+    ap.add_argument("--pin-tiling", action="append", default=[], metavar="STAGE:SHAPE,KCHUNK,XCD",
+                    help="fix a matvec GEMM stage's tiling (counter runs, whose timing would mislead the autotuner)")
     ap.add_argument("--ctm-chi", type=int, default=64)
     ap.add_argument("--ctm-sweeps", type=int, default=5)
     ap.add_argument("--gmres-cycles", type=int, default=1)
@@ -443,6 +452,9 @@ def main():
     from tenax_amd import _lib
 
     pd, prob, (L, Wl, Wr, R, theta), mps, mpo = build_problem(args.L, args.chi, args.seed)
+    for pin in args.pin_tiling:
+        st, cfg = pin.split(":")
+        prob.plan.set_tiling(int(st), *(int(v) for v in cfg.split(",")))
     stats = prob.plan.stats()
     flops = stats["flops"]
     log(f"matvec plan: flops={flops:.3e} pairs={stats['n_pairs']} tasks={stats['n_tasks']}")
@@ -456,8 +468,8 @@ def main():
 
     # per-stage kernel times -> roofline of the dominant kernel
     st_ms, st_fl = stage_times(prob, max(5, args.steps // 2))
-    dom = max((1, 3), key=lambda s: st_ms[s])
-    ach = st_fl[dom] / (st_ms[dom] * 1e-3) / 1e12
+    dom = max((1, 3), key=lambda s: st_ms[f"{s}_gemm"])
+    ach = st_fl[dom] / (st_ms[f"{dom}_gemm"] * 1e-3) / 1e12
     log(f"stages ms {st_ms}  flops {st_fl}")
 
     if args.calibrate_copy:
@@ -527,6 +539,7 @@ def main():
         "roofline": {"bound": "mfma", "kernel": f"{kname} x {kwgs} (matvec stage {dom}, FP64 MFMA)",
                      "achieved": ach, "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
                      "frac": ach / PEAK_FP64_MFMA_TFLOPS, "traffic": traffic,
+                     "kernel_avg_us": st_ms[f"{dom}_gemm"] * 1e3,
                      "traffic_source": "profiles/dmrg_matvec_chi2048_traffic.md (rocprofv3 --pmc FETCH_SIZE x2.0 + "
                                        "WRITE_SIZE x1.0, copy-calibrated)" if traffic else None,
                      "stage_ms": st_ms, "stage_flops": st_fl},

@@ -43,7 +43,10 @@ int tnx_matvec_execute(const tnx_plan *plan, const double *L, const double *Wl, 
 /* One stage of the fused matvec alone (profiling/roofline): stage 0 = MPO
  * coefficient kernel, 1 = L.theta grouped GEMM, 2 = coefficient mix,
  * 3 = .R grouped GEMM.  Stages read the intermediates the previous stage left
- * in the plan, so stage k is meaningful after stages 0..k-1 ran once. */
+ * in the plan, so stage k is meaningful after stages 0..k-1 ran once.
+ * stage | TNX_STAGE_GEMM_ONLY runs the GEMM kernel of stage 1 or 3 without
+ * its split-K reduction (its output is then incomplete: timing only). */
+#define TNX_STAGE_GEMM_ONLY 16
 int tnx_matvec_execute_stage(const tnx_plan *plan, int32_t stage, const double *L,
                              const double *Wl, const double *Wr, const double *R,
                              const double *x, double *y, int32_t compute, void *stream);

@@ -26,6 +26,7 @@ import torch
 CANDIDATES = [(0, 0, 0), (0, 0, 1), (0, -1, 0), (0, -1, 1), (1, 0, 1), (2, 0, 1), (1, -1, 1), (2, -1, 1)]
 MIN_FLOPS = 2e9  # below this a launch is too short for the tiling to matter
 REPS = 2
+TIE = 0.05  # switch away from an earlier candidate only for a clear gain
 ENABLED = os.environ.get("TENAX_AMD_AUTOTUNE", "1") != "0"
 
 _best: dict = {}
@@ -61,7 +62,10 @@ def tune(plan, stage: int, run, flops: float, kind: str):
         for c in CANDIDATES:
             plan.set_tiling(stage, *c)
             timed.append((_time(run, REPS), c))
-        cfg = min(timed)[1]
+        # near-ties go to the earliest candidate (fixed order), so the choice
+        # does not flip between runs on timing noise
+        best = min(t for t, _ in timed)
+        cfg = next(c for t, c in timed if t <= (1.0 + TIE) * best)
         with _lock:
             _best[sig] = cfg
         STATS["timed"] += 1

@@ -1248,23 +1248,24 @@ int tnx_matvec_execute_stage(const tnx_plan *plan, int32_t stage, const double *
     const FusedPlan &P = static_cast<const FusedPlan &>(*plan);
     hipStream_t s = (hipStream_t)stream;
     if (P.n_terms == 0) return;
-    switch (stage) {
+    const bool gemm_only = (stage & TNX_STAGE_GEMM_ONLY) != 0;
+    switch (stage & ~TNX_STAGE_GEMM_ONLY) {
       case 0:
         hipLaunchKernelGGL(coef_kernel, dim3(nb256(P.n_terms)), dim3(256), 0, s, P.n_terms, P.cspec.p, Wl, Wr,
                            P.coef.p);
         break;
       case 1:
-        launch_grouped_gemm(P.s1, L, x, P.X.p, compute, false, s);
+        launch_grouped_gemm(P.s1, L, x, P.X.p, compute, false, s, !gemm_only);
         break;
       case 2:
         hipLaunchKernelGGL(mix_kernel, dim3(P.n_mix_tasks), dim3(256), 0, s, P.items.p, P.mix_titem.p,
                            P.mix_tstart.p, P.mterms.p, P.coef.p, P.X.p, P.Y.p);
         break;
       case 3:
-        launch_grouped_gemm(P.s2, P.Y.p, R, y, compute, false, s);
+        launch_grouped_gemm(P.s2, P.Y.p, R, y, compute, false, s, !gemm_only);
         break;
       default:
-        throw Error{TNX_ERR_ARG, "stage must be 0..3"};
+        throw Error{TNX_ERR_ARG, "stage must be 0..3 (optionally | TNX_STAGE_GEMM_ONLY)"};
     }
     TNX_CHECK_HIP(hipGetLastError());
   });

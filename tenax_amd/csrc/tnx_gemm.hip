@@ -467,7 +467,7 @@ static void launch_any(int a_lay, int b_lay, int shape, int compute, int n_tasks
 }
 
 void launch_grouped_gemm(const GemmProgram &prog, const double *A, const double *B, double *C, int compute,
-                         bool accumulate, hipStream_t s) {
+                         bool accumulate, hipStream_t s, bool reduce) {
   const int acc = accumulate ? 1 : 0;
   if (!prog.tiled) {
     launch_any(prog.a_lay, prog.b_lay, 0, compute, prog.n_tasks, s, prog.tasks.p, prog.outs.p, prog.pairs.p, A, B,
@@ -476,7 +476,7 @@ void launch_grouped_gemm(const GemmProgram &prog, const double *A, const double 
   }
   launch_any(prog.a_lay, prog.b_lay, prog.cfg.shape, compute, prog.n_vtasks, s, prog.vtasks.p, prog.outs.p,
              prog.pairs.p, A, B, C, prog.ws.p, acc);
-  if (prog.n_red > 0) {
+  if (reduce && prog.n_red > 0) {
     const dim3 g(prog.n_red), b(256);
     switch (prog.cfg.shape) {
       case 0: hipLaunchKernelGGL((split_reduce_kernel<64, 64>), g, b, 0, s, prog.red.p, prog.outs.p, prog.ws.p, C, acc); break;

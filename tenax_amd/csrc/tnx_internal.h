@@ -184,8 +184,9 @@ constexpr int64_t MAX_WS_DOUBLES = int64_t(32) << 20;
 void set_tiling(GemmProgram &prog, const TileCfg &cfg, hipStream_t s);
 
 // Launch the grouped GEMM: C[outs] (=|+=) sum_pairs A*B.
+// reduce = false skips the split-K reduction (profiling of the GEMM kernel alone).
 void launch_grouped_gemm(const GemmProgram &prog, const double *A, const double *B, double *C,
-                         int compute, bool accumulate, hipStream_t s);
+                         int compute, bool accumulate, hipStream_t s, bool reduce = true);
 // Launch a prepared program slice (tasks/outs/pairs already on the device).
 void launch_prepared(int a_lay, int b_lay, int compute, int n_tasks, hipStream_t s, const GemmTask *tasks,
                      const GemmOut *outs, const GemmPair *pairs, const double *A, const double *B, double *C,
