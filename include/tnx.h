@@ -95,6 +95,20 @@ int tnx_plan_pairwise(const tnx_blocks *a, const tnx_blocks *b, int32_t a_kfirst
                       int32_t has_target, int32_t target, void *stream,
                       tnx_plan **plan);
 
+/* Pairwise plan for graded (fermionic) charges: as tnx_plan_pairwise, and
+ * each block pair enters with the exchange sign of the reference
+ * (contractor.py:206-285 inversion pairs, applied per combination at
+ * 462-470): sign_pairs holds n_sign_pairs pairs (x, y) of leg positions in
+ * the concatenated [A legs..., B legs...] of the operands as passed; a pair
+ * whose two charges are both odd (parity = charge & 1, FermionParity and
+ * FermionicU1) flips the sign of the block product.  The caller derives the
+ * pairs from the contraction's subscripts. */
+int tnx_plan_pairwise_graded(const tnx_blocks *a, const tnx_blocks *b, int32_t a_kfirst,
+                             int32_t b_kfirst, int32_t n_contracted, const tnx_leg *out_legs,
+                             int32_t sym_n, int32_t mode, int32_t has_target, int32_t target,
+                             int32_t n_sign_pairs, const int32_t *sign_pairs, void *stream,
+                             tnx_plan **plan);
+
 /* Output structure of a plan (pairwise / transpose plans). */
 int tnx_plan_output_info(const tnx_plan *plan, int32_t *n_blocks, int32_t *rank,
                          int64_t *total_elems);
@@ -142,6 +156,11 @@ int tnx_plan_transpose(const tnx_blocks *a, const tnx_leg *legs, const int32_t *
                        void *stream, tnx_plan **plan);
 int tnx_transpose_execute(const tnx_plan *plan, const double *a, double *out,
                           void *stream);
+/* Leg permutation of a graded (fermionic) tensor, graded != 0: each block is
+ * multiplied by its Koszul sign (tensor.py:32-50, 853-856: -1 per inversion
+ * of the permutation between two legs with odd charges, parity = charge & 1). */
+int tnx_plan_transpose_graded(const tnx_blocks *a, const tnx_leg *legs, const int32_t *perm,
+                              int32_t graded, void *stream, tnx_plan **plan);
 
 #ifdef __cplusplus
 }

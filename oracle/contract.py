@@ -14,7 +14,7 @@ from collections import Counter
 
 import numpy as np
 
-from oracle.core import SymT, compute_valid_blocks
+from oracle.core import SymT, compute_valid_blocks, is_fermionic, parity
 
 _CHARS = string.ascii_lowercase + string.ascii_uppercase
 
@@ -52,8 +52,30 @@ def labels_to_subscripts(tensors, output_labels=None):
     return s, tuple(first[l] for l in output_labels)
 
 
+def inversion_pairs(input_subs, output_part):
+    """contractor.py:206-285: the concatenated input legs are sent to
+    [output legs..., contracted legs (first occurrence, second occurrence) in
+    order of first appearance]; every inversion is a character pair."""
+    chars = [c for s in input_subs for c in s]
+    cnt = Counter(chars)
+    contracted = {c for c, n in cnt.items() if n >= 2}
+    target = list(output_part)
+    for c in chars:
+        if c in contracted and c not in target:
+            target.append(c)
+    pos = {c: i for i, c in enumerate(target)}
+    used = Counter()
+    ranks = []
+    for c in chars:
+        ranks.append(2 * pos[c] + (used[c] if c in contracted else 0))
+        used[c] += 1
+    return [(chars[i], chars[j]) for i in range(len(chars)) for j in range(i + 1, len(chars))
+            if ranks[i] > ranks[j]]
+
+
 def contract_symmetric(tensors, subscripts):
-    """contractor.py:291-484 (bosonic): signature buckets, product, filter.
+    """contractor.py:291-484: signature buckets, product, filter, and the
+    fermionic sign of each combination (462-470).
 
     Returns (SymT, combos) where combos lists (output_key, input keys) in the
     order the reference loop visits them (common signatures in sorted order
@@ -77,6 +99,9 @@ def contract_symmetric(tensors, subscripts):
                 total += ts.pop()
     target = total if total != 0 else None
     valid = set(compute_valid_blocks(out_legs, target))
+    sym0 = tensors[0].legs[0].sym
+    ferm = is_fermionic(sym0)
+    inv = inversion_pairs(isubs, outp) if ferm else []
     by_sig = []
     for t, s in zip(tensors, isubs):
         pos = sorted(((contracted.index(c), p) for p, c in enumerate(s) if c in contracted))
@@ -107,6 +132,12 @@ def contract_symmetric(tensors, subscripts):
             if okey not in valid:
                 continue
             r = _einsum(subscripts, [a for _, a in combo])
+            if ferm:
+                sg = 1
+                for ci, cj in inv:
+                    if parity(sym0, cc[ci]) and parity(sym0, cc[cj]):
+                        sg = -sg
+                r = r * sg
             combos.append((okey, tuple(k for k, _ in combo)))
             out[okey] = out[okey] + r if okey in out else r
     return SymT(out, out_legs, validate=target is None), combos

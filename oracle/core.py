@@ -1,7 +1,8 @@
 """Oracle: charge algebra, leg metadata and dict-of-blocks symmetric tensors.
 
-Restates reference src/tenax/core/symmetry.py:182-255, core/index.py:27-195
-and core/tensor.py:53-188, 296-319, 505-974 in NumPy FP64.
+Restates reference src/tenax/core/symmetry.py:182-255, 311-440,
+core/index.py:27-195 and core/tensor.py:32-50, 53-188, 296-319, 505-974 in
+NumPy FP64.
 Test infrastructure only (see oracle/__init__.py).
 """
 
@@ -51,6 +52,53 @@ class Zn:
 
     def __hash__(self):
         return hash(("Zn", self.n))
+
+
+class FermionParity(Zn):
+    """core/symmetry.py:311-371: Z2 with fermionic exchange; parity = q."""
+
+    fermionic = True
+
+    def __init__(self):
+        super().__init__(2)
+
+    def __eq__(self, o):
+        return isinstance(o, FermionParity)
+
+    def __hash__(self):
+        return hash("FermionParity")
+
+
+class FermionicU1(U1):
+    """core/symmetry.py:374-440: U(1) with parity |q| mod 2 (default grading)."""
+
+    fermionic = True
+
+    def __eq__(self, o):
+        return isinstance(o, FermionicU1)
+
+    def __hash__(self):
+        return hash("FermionicU1")
+
+
+def is_fermionic(sym) -> bool:
+    return bool(getattr(sym, "fermionic", False))
+
+
+def parity(sym, q) -> int:
+    """symmetry.py:99-109, 345-346, 426-427: 0 for bosonic groups, |q| mod 2 otherwise."""
+    return abs(int(q)) % 2 if is_fermionic(sym) else 0
+
+
+def koszul_sign(parities, perm) -> int:
+    """tensor.py:32-50: -1 per inversion of ``perm`` between two odd elements
+    (parities indexed by the ORIGINAL positions)."""
+    sign = 1
+    for i in range(len(perm)):
+        for j in range(i + 1, len(perm)):
+            if perm[i] > perm[j] and parities[perm[i]] and parities[perm[j]]:
+                sign = -sign
+    return sign
 
 
 def fuse_many(sym, qs):
@@ -231,9 +279,13 @@ class SymT:
         return SymT({k: np.conj(v) for k, v in self.blocks.items()}, self.legs, False)
 
     def dagger(self):
-        # tensor.py:783-816 (bosonic: dual keys and legs, conj data)
+        # tensor.py:783-816: dual keys and legs, conj data, and for fermionic
+        # groups the product of twist phases (-1)^parity(q) over the key
         sym = self.legs[0].sym
-        nb = {tuple(int(sym.dual(np.int64(q))) for q in k): np.conj(v) for k, v in self.blocks.items()}
+        nb = {}
+        for k, v in self.blocks.items():
+            tw = (-1) ** sum(parity(sym, q) for q in k)
+            nb[tuple(int(sym.dual(np.int64(q))) for q in k)] = np.conj(v) * tw
         return SymT(nb, tuple(l.dual() for l in self.legs), False)
 
     def bar(self):
@@ -242,8 +294,12 @@ class SymT:
                     tuple(l.flip_flow() for l in self.legs), False)
 
     def transpose(self, axes):
-        # tensor.py:832-861 (bosonic)
-        nb = {tuple(k[i] for i in axes): np.transpose(v, axes) for k, v in self.blocks.items()}
+        # tensor.py:832-861: fermionic blocks take their Koszul sign (853-856)
+        sym = self.legs[0].sym
+        nb = {}
+        for k, v in self.blocks.items():
+            sg = koszul_sign([parity(sym, q) for q in k], axes) if is_fermionic(sym) else 1
+            nb[tuple(k[i] for i in axes)] = np.transpose(v, axes) * sg
         return SymT(nb, tuple(self.legs[i] for i in axes), False)
 
     def norm(self):

@@ -10,7 +10,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from oracle.core import IN, OUT, Leg, SymT, fuse_many
+from oracle.core import IN, OUT, Leg, SymT, fuse_many, is_fermionic, koszul_sign, parity
 
 
 def group_blocks_by_bond_charge(t, left_axes, right_axes):
@@ -49,10 +49,20 @@ def truncated_svd_symmetric(t, left_labels, right_labels, new_bond_label,
         if R == 0 or C == 0:
             continue
         M = np.zeros((R, C))
+        sym = t.legs[0].sym
         for lk, rk, blk in entries:
             li, ri = lseen[lk], rseen[rk]
             r0, c0 = sum(lsz[:li]), sum(rsz[:ri])
-            M[r0:r0 + lsz[li], c0:c0 + rsz[ri]] = blk.reshape(lsz[li], rsz[ri])
+            b = blk.reshape(lsz[li], rsz[ri])
+            if is_fermionic(sym):
+                # linalg.py:171-186: Koszul sign of the reordering to (left, right)
+                full = [0] * len(t.legs)
+                for ax, c in zip(la, lk):
+                    full[ax] = c
+                for ax, c in zip(ra, rk):
+                    full[ax] = c
+                b = b * koszul_sign([parity(sym, c) for c in full], la + ra)
+            M[r0:r0 + lsz[li], c0:c0 + rsz[ri]] = b
         if driver == "gesdd":
             U, s, Vh = np.linalg.svd(M, full_matrices=False)
         else:

@@ -22,6 +22,8 @@ from tenax_amd.core import (
     BaseSymmetry,
     BlockKey,
     DenseTensor,
+    FermionicU1,
+    FermionParity,
     FlowDirection,
     Label,
     PRNGKey,

@@ -75,6 +75,8 @@ SIGNATURES = {
     "tnx_valid_blocks": (C.c_int, [C.POINTER(Leg), _i32, _i32, _i32, _i32, _i32, _PI32, _PI32, _PI32, _p]),
     "tnx_plan_pairwise": (C.c_int, [C.POINTER(Blocks), C.POINTER(Blocks), _i32, _i32, _i32, C.POINTER(Leg),
                                     _i32, _i32, _i32, _i32, _p, C.POINTER(_p)]),
+    "tnx_plan_pairwise_graded": (C.c_int, [C.POINTER(Blocks), C.POINTER(Blocks), _i32, _i32, _i32, C.POINTER(Leg),
+                                           _i32, _i32, _i32, _i32, _i32, _PI32, _p, C.POINTER(_p)]),
     "tnx_plan_output_info": (C.c_int, [_p, _PI32, _PI32, _PI64]),
     "tnx_plan_output_copy": (C.c_int, [_p, _PI32, _PI32, _PI64]),
     "tnx_plan_stats": (C.c_int, [_p, _PD, _PD, _PI32, _PI32]),
@@ -85,6 +87,7 @@ SIGNATURES = {
     "tnx_contract_execute_perblock": (C.c_int, [_p, _p, _p, _p, _i32, _p]),
     "tnx_plan_transpose": (C.c_int, [C.POINTER(Blocks), C.POINTER(Leg), _PI32, _p, C.POINTER(_p)]),
     "tnx_transpose_execute": (C.c_int, [_p, _p, _p, _p]),
+    "tnx_plan_transpose_graded": (C.c_int, [C.POINTER(Blocks), C.POINTER(Leg), _PI32, _i32, _p, C.POINTER(_p)]),
     # include/tnx_dmrg.h
     "tnx_matvec_plan": (C.c_int, [C.POINTER(Blocks)] * 5 + [_p, C.POINTER(_p)]),
     "tnx_matvec_reach": (C.c_int, [_p, _PI32, _PI32, _PI32, _i32]),

@@ -34,7 +34,7 @@ import torch
 from tenax_amd import _lib
 from tenax_amd.contraction import plans
 from tenax_amd.core.index import Label, TensorIndex
-from tenax_amd.core.tensor import DenseTensor, SymmetricTensor, Tensor
+from tenax_amd.core.tensor import DenseTensor, SymmetricTensor, Tensor, _transpose_raw
 
 _CHARS = string.ascii_lowercase + string.ascii_uppercase
 
@@ -188,12 +188,40 @@ class _PairExecFn(torch.autograd.Function):
         return ga, gb, None, None, None, None, None, None, None
 
 
+def _inversion_pairs(input_subs: Sequence[str], output: str) -> list[tuple[str, str]]:
+    """Leg pairs whose exchange can sign a fermionic contraction
+    (contractor.py:206-285).  The input legs, concatenated, are carried to
+    the order [output legs..., each contracted leg's two occurrences] (first
+    occurrence, then second, contracted legs in order of first appearance);
+    every inversion of that reordering is returned as a pair of subscript
+    characters.  A block combination takes a factor -1 for each pair whose two
+    charges are both odd."""
+    seq = "".join(input_subs)
+    counts = Counter(seq)
+    contracted = list(dict.fromkeys(c for c in seq if counts[c] >= 2))
+    slot = {c: i for i, c in enumerate(output)}
+    for i, c in enumerate(contracted):
+        slot[c] = len(output) + i
+    rank, seen = [], Counter()
+    for c in seq:
+        rank.append(2 * slot[c] + (seen[c] if counts[c] >= 2 else 0))
+        seen[c] += 1
+    return [(seq[i], seq[j]) for i in range(len(seq)) for j in range(i + 1, len(seq)) if rank[i] > rank[j]]
+
+
+def _is_graded(T) -> bool:
+    return bool(T.indices) and T.indices[0].symmetry.is_fermionic
+
+
 def contract_pair(A: SymmetricTensor, B: SymmetricTensor, sa: str, sb: str, so: str,
                   out_indices: tuple[TensorIndex, ...], mode: int, target) -> SymmetricTensor:
     """One pairwise block-sparse contraction with einsum-style subscripts.
 
     ``out_indices`` are the legs of ``so`` in order.  mode = filtered
     (conservation, contractor.py semantics) or reachable (dmrg.py:1048-1151).
+    For graded (fermionic) groups each block pair takes the exchange sign of
+    the reference (contractor.py:462-470), evaluated on the device; the
+    operand and output reshuffles below then move data without signs.
     """
     if len(set(sa)) != len(sa) or len(set(sb)) != len(sb):
         raise NotImplementedError("traces within one operand are not supported")
@@ -202,9 +230,17 @@ def contract_pair(A: SymmetricTensor, B: SymmetricTensor, sa: str, sb: str, so: 
     free_b = [c for c in sb if c not in cont]
     if sorted(so) != sorted(free_a + free_b):
         raise ValueError(f"output subscripts {so!r} are not the free legs of {sa!r},{sb!r}")
+    graded = _is_graded(A)
+    sign_chars = []
+    if graded:
+        if mode != _lib.MODE_PAIRWISE_FILTERED:
+            raise NotImplementedError("fermionic signs are defined for the contract() semantics only")
+        if torch.is_grad_enabled() and (A.data.requires_grad or B.data.requires_grad):
+            raise NotImplementedError("autograd through fermionic block contractions is not supported")
+        sign_chars = _inversion_pairs([sa, sb], so)
     ix_of = dict(zip(so, out_indices))
     if not so:
-        return _full_contraction(A, B, sa, sb)
+        return _full_contraction(A, B, sa, sb, sign_chars)
     # Let the operand whose free leg comes first in the output play "A".
     if free_a and free_b and so[0] in free_b:
         A, B, sa, sb = B, A, sb, sa
@@ -213,22 +249,26 @@ def contract_pair(A: SymmetricTensor, B: SymmetricTensor, sa: str, sb: str, so: 
     fa = [c for c in so if c in free_a]
     fb = [c for c in so if c in free_b]
     nat = "".join(fa + fb)
+    move = (lambda T, axes: _transpose_raw(T, axes, graded=False)) if graded else (lambda T, axes: T.transpose(axes))
     # A layout
     if list(sa) == fa + cont:
         a_kfirst = 0
     elif list(sa) == cont + fa:
         a_kfirst = 1
     else:
-        A = A.transpose(tuple(sa.index(c) for c in fa + cont))
+        A = move(A, tuple(sa.index(c) for c in fa + cont))
         sa, a_kfirst = "".join(fa + cont), 0
     # B layout: always [contracted (in A's order)..., free...]
     b_kfirst = 1
     if list(sb) != cont + fb:
-        B = B.transpose(tuple(sb.index(c) for c in cont + fb))
+        B = move(B, tuple(sb.index(c) for c in cont + fb))
         sb = "".join(cont + fb)
     nat_indices = tuple(ix_of[c] for c in nat)
+    pos = {c: i for i, c in enumerate(sa)}
+    for i, c in enumerate(sb):
+        pos.setdefault(c, len(sa) + i)
     plan, keys, shapes, offs, tot = plans.pairwise_plan(A, B, a_kfirst, b_kfirst, len(cont), nat_indices, mode,
-                                                        target)
+                                                        target, [(pos[x], pos[y]) for x, y in sign_chars])
     out = torch.empty(tot, dtype=torch.float64, device=A.data.device)
     C = SymmetricTensor._wrap(nat_indices, keys, shapes, offs, out)
     if torch.is_grad_enabled() and (A.data.requires_grad or B.data.requires_grad):
@@ -238,18 +278,21 @@ def contract_pair(A: SymmetricTensor, B: SymmetricTensor, sa: str, sb: str, so: 
     elif tot:
         _execute_pair(plan, A, B, out)
     if nat != so:
-        C = C.transpose(tuple(nat.index(c) for c in so))
+        C = move(C, tuple(nat.index(c) for c in so))
     return C
 
 
-def _full_contraction(A, B, sa, sb):
-    Bp = B if sb == sa else B.transpose(tuple(sb.index(c) for c in sa))
+def _full_contraction(A, B, sa, sb, sign_chars=()):
+    Bp = B if sb == sa else _transpose_raw(B, tuple(sb.index(c) for c in sa), graded=False)
     ka, kb = A.key_index(), Bp.key_index()
     tot = torch.zeros((), dtype=torch.float64, device=A.data.device)
     for k, i in ka.items():
         j = kb.get(k)
         if j is not None:
-            tot = tot + torch.dot(A.block(i).reshape(-1), Bp.block(j).reshape(-1))
+            q = dict(zip(sa, k))
+            neg = sum((q[x] & 1) * (q[y] & 1) for x, y in sign_chars) % 2
+            d = torch.dot(A.block(i).reshape(-1), Bp.block(j).reshape(-1))
+            tot = tot - d if neg else tot + d
     return SymmetricTensor._wrap((), np.zeros((1, 0), np.int32), np.zeros((1, 0), np.int32),
                                  np.zeros(1, np.int64), tot.reshape(1))
 
@@ -273,7 +316,10 @@ def _contract_symmetric(tensors, subscripts, output_indices, optimize="auto"):
             total += u
     target = total if total != 0 else None
     dims = {c: ix.dim for c, ix in c2ix.items()}
-    pairs = plan_order(subs, dims) if optimize != "left" else [(0, 1)] * (len(tensors) - 1)
+    # graded groups fold left to right: each step's exchange sign is then
+    # relative to the operands' own order
+    left = optimize == "left" or _is_graded(tensors[0])
+    pairs = [(0, 1)] * (len(tensors) - 1) if left else plan_order(subs, dims)
     items = list(zip(tensors, subs))
     for i, j in pairs:
         (a, sa), (b, sb) = items[i], items[j]

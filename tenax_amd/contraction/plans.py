@@ -51,11 +51,15 @@ def _legs_sig(indices):
     return tuple((ix.sectors()[0].tobytes(), ix.sectors()[1].tobytes(), int(ix.flow)) for ix in indices)
 
 
-def pairwise_plan(A, B, a_kfirst: int, b_kfirst: int, nc: int, out_indices, mode: int, target):
-    """Plan C = A . B (device planner); returns (Plan, keys, shapes, offsets, total)."""
+def pairwise_plan(A, B, a_kfirst: int, b_kfirst: int, nc: int, out_indices, mode: int, target, sign_pairs=()):
+    """Plan C = A . B (device planner); returns (Plan, keys, shapes, offsets, total).
+
+    ``sign_pairs``: fermionic exchange-sign leg pairs (positions in the
+    concatenated [A legs, B legs]), evaluated per block pair on the device."""
     sym = A.indices[0].symmetry if A.indices else out_indices[0].symmetry
+    sign_pairs = tuple((int(x), int(y)) for x, y in sign_pairs)
     key = ("pair", _struct_sig(A), _struct_sig(B), A.ndim, B.ndim, a_kfirst, b_kfirst, nc,
-           _legs_sig(out_indices), sym.sym_n, mode, target)
+           _legs_sig(out_indices), sym.sym_n, mode, target, sign_pairs)
 
     def build():
         lib = _lib.load()
@@ -64,9 +68,11 @@ def pairwise_plan(A, B, a_kfirst: int, b_kfirst: int, nc: int, out_indices, mode
         legs = _lib.LegArray([(*ix.sectors(), int(ix.flow)) for ix in out_indices])
         ptr = _lib.C.c_void_p()
         has_t = target is not None
-        rc = lib.tnx_plan_pairwise(_lib.C.byref(ta.s), _lib.C.byref(tb.s), a_kfirst, b_kfirst, nc, legs.arr,
-                                   sym.sym_n, mode, int(has_t), int(target) if has_t else 0,
-                                   _lib.stream_handle(), _lib.C.byref(ptr))
+        sp = np.asarray(sign_pairs, np.int32).reshape(-1)
+        rc = lib.tnx_plan_pairwise_graded(_lib.C.byref(ta.s), _lib.C.byref(tb.s), a_kfirst, b_kfirst, nc, legs.arr,
+                                          sym.sym_n, mode, int(has_t), int(target) if has_t else 0,
+                                          len(sign_pairs), _lib.i32p(sp) if len(sp) else None,
+                                          _lib.stream_handle(), _lib.C.byref(ptr))
         _lib.check(rc)
         plan = _lib.Plan(ptr)
         keys, shapes, offs, tot = plan.output()
@@ -75,8 +81,9 @@ def pairwise_plan(A, B, a_kfirst: int, b_kfirst: int, nc: int, out_indices, mode
     return _get(key, build)
 
 
-def transpose_plan(T, axes):
-    key = ("transpose", _struct_sig(T), T.ndim, tuple(axes), _legs_sig(T.indices))
+def transpose_plan(T, axes, graded: bool = False):
+    """Leg permutation plan; ``graded``: multiply each block by its Koszul sign."""
+    key = ("transpose", _struct_sig(T), T.ndim, tuple(axes), _legs_sig(T.indices), bool(graded))
 
     def build():
         lib = _lib.load()
@@ -84,8 +91,8 @@ def transpose_plan(T, axes):
         legs = _lib.LegArray([(*ix.sectors(), int(ix.flow)) for ix in T.indices])
         perm = np.asarray(axes, np.int32)
         ptr = _lib.C.c_void_p()
-        _lib.check(lib.tnx_plan_transpose(_lib.C.byref(ta.s), legs.arr, _lib.i32p(perm), _lib.stream_handle(),
-                                          _lib.C.byref(ptr)))
+        _lib.check(lib.tnx_plan_transpose_graded(_lib.C.byref(ta.s), legs.arr, _lib.i32p(perm), int(bool(graded)),
+                                                 _lib.stream_handle(), _lib.C.byref(ptr)))
         plan = _lib.Plan(ptr)
         keys, shapes, offs, tot = plan.output()
         return plan, keys, shapes, offs, tot

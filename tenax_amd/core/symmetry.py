@@ -1,9 +1,12 @@
 """Charge algebra of abelian symmetries (host metadata, NumPy integers).
 
-Mirrors the public interface of reference src/tenax/core/symmetry.py:29-255:
-``BaseSymmetry.fuse/dual/identity/n_values/fuse_many/is_conserved``, ``U1Symmetry``
-(fuse = +, dual = -q, unbounded) and ``ZnSymmetry`` (fuse = + mod n).  The
-device planner receives the group as ``sym_n`` (0 for U(1), n for Z_n).
+Mirrors the public interface of reference src/tenax/core/symmetry.py:29-255
+and 311-440: ``BaseSymmetry.fuse/dual/identity/n_values/fuse_many/
+is_conserved``, ``U1Symmetry`` (fuse = +, dual = -q, unbounded), ``ZnSymmetry``
+(fuse = + mod n), and the graded groups ``FermionParity`` (Z2, parity = q)
+and ``FermionicU1`` (U(1), parity = |q| mod 2).  The device planner receives
+the group as ``sym_n`` (0 for U(1), n for Z_n) and, for the graded groups,
+applies exchange signs with parity = charge & 1.
 """
 
 from __future__ import annotations
@@ -48,6 +51,16 @@ class BaseSymmetry(ABC):
 
     def parity(self, charges):
         return np.zeros_like(charges)
+
+    def exchange_sign(self, charge_a: int, charge_b: int) -> int:
+        """Sign of exchanging two objects of these charges (symmetry.py:110-128)."""
+        pa = int(self.parity(np.array([charge_a]))[0])
+        pb = int(self.parity(np.array([charge_b]))[0])
+        return 1 - 2 * (pa * pb)
+
+    def twist_phase(self, charge: int) -> float:
+        """(-1)^parity (symmetry.py:130-145); 1 for bosonic groups."""
+        return -1.0 if int(self.parity(np.array([charge]))[0]) else 1.0
 
     def is_conserved(self, charges_per_leg, flows, target=None) -> bool:
         want = self.identity() if target is None else target
@@ -113,3 +126,82 @@ class ZnSymmetry(BaseSymmetry):
 
     def __repr__(self):
         return f"ZnSymmetry({self.n})"
+
+
+class FermionParity(BaseSymmetry):
+    """Z2 with fermionic exchange statistics: charge 0 even, 1 odd
+    (symmetry.py:311-371)."""
+
+    @property
+    def is_fermionic(self) -> bool:
+        return True
+
+    def fuse(self, charges_a, charges_b):
+        return (charges_a + charges_b) % 2
+
+    def dual(self, charges):
+        return charges
+
+    def identity(self) -> int:
+        return 0
+
+    def n_values(self) -> int:
+        return 2
+
+    def parity(self, charges):
+        return np.asarray(np.asarray(charges) % 2, dtype=np.int32)
+
+    def __eq__(self, other):
+        return isinstance(other, FermionParity)
+
+    def __hash__(self):
+        return hash("FermionParity")
+
+    def __repr__(self):
+        return "FermionParity()"
+
+
+_GRADINGS = ("abs_mod_2", "mod_2")
+
+
+class FermionicU1(BaseSymmetry):
+    """U(1) with fermionic statistics, parity = |q| mod 2 (symmetry.py:374-440).
+
+    The grading keys of the reference ("abs_mod_2", "mod_2") coincide on the
+    integers; callable gradings are not supported (the device evaluates the
+    parity as charge & 1)."""
+
+    def __init__(self, grading=None, grading_key: str = "abs_mod_2"):
+        if grading is not None:
+            raise NotImplementedError("callable gradings are not supported; use grading_key")
+        if grading_key not in _GRADINGS:
+            raise ValueError(f"Unknown grading_key: {grading_key!r}")
+        self._grading_key = grading_key
+
+    @property
+    def is_fermionic(self) -> bool:
+        return True
+
+    def fuse(self, charges_a, charges_b):
+        return charges_a + charges_b
+
+    def dual(self, charges):
+        return -charges
+
+    def identity(self) -> int:
+        return 0
+
+    def n_values(self):
+        return None
+
+    def parity(self, charges):
+        return np.asarray(np.abs(np.asarray(charges)) % 2, dtype=np.int32)
+
+    def __eq__(self, other):
+        return isinstance(other, FermionicU1) and other._grading_key == self._grading_key
+
+    def __hash__(self):
+        return hash(("FermionicU1", self._grading_key))
+
+    def __repr__(self):
+        return f"FermionicU1(grading_key={self._grading_key!r})"

@@ -231,9 +231,10 @@ class SymmetricTensor(Tensor):
     def _wrap(cls, indices, keys, shapes, offsets, data) -> "SymmetricTensor":
         obj = object.__new__(cls)
         obj._indices = tuple(indices)
-        obj._keys = np.asarray(keys, np.int32).reshape(-1, len(obj._indices))
-        obj._shapes = np.asarray(shapes, np.int32).reshape(-1, len(obj._indices))
         obj._offsets = np.asarray(offsets, np.int64).reshape(-1)
+        n = len(obj._offsets)  # one offset per block (rank 0: the scalar block)
+        obj._keys = np.asarray(keys, np.int32).reshape(n, len(obj._indices))
+        obj._shapes = np.asarray(shapes, np.int32).reshape(n, len(obj._indices))
         obj._data = data
         obj._kidx = None
         return obj
@@ -360,13 +361,21 @@ class SymmetricTensor(Tensor):
         return self._with_data(self._data.conj(), tuple(ix.flip_flow() for ix in self._indices))
 
     def dagger(self):
-        """Dual legs, dual keys, conjugated data (tensor.py:783-816, bosonic)."""
+        """Dual legs, dual keys, conjugated data (tensor.py:783-816); graded
+        groups multiply each block by the product of its charges' twist
+        phases, (-1)^(number of odd charges)."""
         sym = self._indices[0].symmetry
         new_keys = sym.dual(self._keys.astype(np.int64)).astype(np.int32)
-        return _reorder(self, tuple(ix.dual() for ix in self._indices), new_keys, conj=True)
+        t = self
+        if sym.is_fermionic and self.n_blocks:
+            odd = np.asarray(sym.parity(self._keys.reshape(-1)), np.int64).reshape(self._keys.shape).sum(axis=1) % 2
+            if odd.any():
+                t = self._with_data(self._data * _block_sign_vector(self, odd))
+        return _reorder(t, tuple(ix.dual() for ix in self._indices), new_keys, conj=True)
 
     def transpose(self, axes):
-        """Leg permutation through the device transpose plan (tensor.py:832-861).
+        """Leg permutation through the device transpose plan (tensor.py:832-861);
+        for graded (fermionic) groups each block takes its Koszul sign.
 
         Differentiable: when the data carries a gradient the kernel runs
         behind an autograd node whose adjoint is the inverse permutation."""
@@ -376,7 +385,7 @@ class SymmetricTensor(Tensor):
         if torch.is_grad_enabled() and self._data.requires_grad:
             from tenax_amd.contraction.plans import transpose_plan
 
-            _, keys, shapes, offs, _ = transpose_plan(self, axes)
+            _, keys, shapes, offs, _ = transpose_plan(self, axes, _graded(self))
             out = _TransposeFn.apply(self._data, self, axes)
             return SymmetricTensor._wrap(tuple(self._indices[i] for i in axes), keys, shapes, offs, out)
         return _transpose_raw(self, axes)
@@ -419,10 +428,25 @@ class SymmetricTensor(Tensor):
                 f"dtype={self.dtype}, labels={self.labels()})")
 
 
-def _transpose_raw(t: "SymmetricTensor", axes) -> "SymmetricTensor":
+def _graded(t: "SymmetricTensor") -> bool:
+    return bool(t._indices) and t._indices[0].symmetry.is_fermionic
+
+
+def _block_sign_vector(t: "SymmetricTensor", negative) -> torch.Tensor:
+    """+-1 per element of t's packed data, -1 on the blocks flagged in ``negative``."""
+    sizes = np.prod(t._shapes, axis=1).astype(np.int64) if t.n_blocks else np.zeros(0, np.int64)
+    per = np.repeat(np.where(np.asarray(negative, bool), -1.0, 1.0), sizes)
+    return torch.from_numpy(per).to(t._data.device)
+
+
+def _transpose_raw(t: "SymmetricTensor", axes, graded=None) -> "SymmetricTensor":
+    """Device leg permutation; ``graded`` (default: the group is fermionic)
+    applies the Koszul signs, ``graded=False`` moves data only."""
     from tenax_amd.contraction.plans import transpose_plan
 
-    plan, keys, shapes, offs, tot = transpose_plan(t, axes)
+    if graded is None:
+        graded = _graded(t)
+    plan, keys, shapes, offs, tot = transpose_plan(t, axes, graded)
     out = torch.empty(tot, dtype=torch.float64, device=t._data.device)
     if t.n_blocks:
         lib = _lib.load()

@@ -30,6 +30,7 @@ struct Stage {
   static constexpr int PB = BN / 64;
   double a[PA][4];
   double b[PB][4];
+  double sgn;  // -1 for a pair with a fermionic exchange sign, applied to A
 };
 
 // Operand loads go through a buffer resource spanning the one block: an
@@ -117,7 +118,7 @@ __device__ __forceinline__ void store_tiles(T *As, T *Bs, const Stage<BM, BN> &s
   for (int q = 0; q < BM / 64; ++q)
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      As[a_lds<ALAY, BM>(tid, q, i)] = (T)st.a[q][i];
+      As[a_lds<ALAY, BM>(tid, q, i)] = (T)(st.a[q][i] * st.sgn);
 #pragma unroll
   for (int q = 0; q < BN / 64; ++q)
 #pragma unroll
@@ -133,7 +134,7 @@ __device__ __forceinline__ void store_tiles_split(float *Ah, float *Al, float *B
   for (int q = 0; q < BM / 64; ++q)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const double x = st.a[q][i];
+      const double x = st.a[q][i] * st.sgn;
       const float h = (float)x;
       const int at = a_lds<ALAY, BM>(tid, q, i);
       Ah[at] = h;
@@ -346,6 +347,7 @@ __global__ __launch_bounds__(GEMM_THREADS) __attribute__((amdgpu_waves_per_eu(
   auto load = [&](Stage<BM, BN> &st) {
     load_a<ALAY, BM>(A, p, o.M, t.m0, lk0, tid, st.a);
     load_b<BN>(B, p, o.N, t.n0, lk0, tid, st.b);
+    st.sgn = p.neg ? -1.0 : 1.0;
     advance();
   };
 

@@ -99,7 +99,7 @@ struct GemmPair {
   int32_t K;
   int32_t lda;  // leading dimension of the A block in its layout
   int32_t ldb;  // leading dimension of the B block in its layout
-  int32_t pad;
+  int32_t neg;  // 1: the product enters with a minus sign (fermionic exchange sign)
 };
 
 // One output block: C (M x N, row-major, ld = ldc) = sum over its pairs.

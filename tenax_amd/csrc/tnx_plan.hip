@@ -315,10 +315,27 @@ __global__ void segments_kernel(int64_t n, const uint32_t *__restrict__ soid, in
   if (k == n - 1 || soid[k + 1] != o) pend[o] = (int32_t)(k + 1);
 }
 
+// Fermionic exchange sign of a block pair (contractor.py:462-470): one factor
+// -1 per listed leg pair (x, y) -- positions in the concatenated [A legs, B
+// legs] -- whose two charges are both odd (parity = charge & 1).
+__device__ __forceinline__ int pair_negative(const int32_t *ka, const int32_t *kb, int rA, const int32_t *sgn,
+                                             int n_sgn) {
+  int neg = 0;
+  for (int t = 0; t < n_sgn; ++t) {
+    const int x = sgn[2 * t], y = sgn[2 * t + 1];
+    const int32_t qx = x < rA ? ka[x] : kb[x - rA];
+    const int32_t qy = y < rA ? ka[y] : kb[y - rA];
+    neg ^= (qx & 1) & (qy & 1);
+  }
+  return neg;
+}
+
 __global__ void build_pairs_kernel(int64_t n, PairSpec sp, int a_kfirst, int b_kfirst,
                                    const uint32_t *__restrict__ sidx, const int32_t *__restrict__ sA,
                                    const int32_t *__restrict__ sB, const int64_t *__restrict__ offA,
-                                   const int64_t *__restrict__ offB, GemmPair *__restrict__ pairs) {
+                                   const int64_t *__restrict__ offB, const int32_t *__restrict__ kA,
+                                   const int32_t *__restrict__ kB, const int32_t *__restrict__ sgn, int n_sgn,
+                                   GemmPair *__restrict__ pairs) {
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (k >= n) return;
   const int64_t p = sidx[k];
@@ -334,7 +351,7 @@ __global__ void build_pairs_kernel(int64_t n, PairSpec sp, int a_kfirst, int b_k
   g.lda = a_kfirst ? (int32_t)M : (int32_t)K;
   g.ldb = (int32_t)N;
   (void)b_kfirst;
-  g.pad = 0;
+  g.neg = n_sgn ? pair_negative(kA + (int64_t)i * sp.rA, kB + (int64_t)j * sp.rB, sp.rA, sgn, n_sgn) : 0;
   pairs[k] = g;
 }
 
@@ -554,7 +571,7 @@ struct PairwisePlan : public tnx_plan_s {
 
 struct TDesc {
   int64_t in_off, out_off, size;
-  int32_t rank, pad;
+  int32_t rank, neg;  // neg: the block's Koszul sign is -1 (graded transpose)
   int32_t shape[MAX_RANK];
   int64_t stride[MAX_RANK];  // input stride of output dim d
 };
@@ -568,8 +585,12 @@ struct TransposePlan : public tnx_plan_s {
 
 static void build_pairwise(PairwisePlan &P, const tnx_blocks *a, const tnx_blocks *b, int a_kfirst, int b_kfirst,
                            int nc, const tnx_leg *out_legs, int sym_n, int mode, bool has_target, int target,
-                           hipStream_t s) {
+                           hipStream_t s, int n_sign = 0, const int32_t *sign_pairs = nullptr) {
   TNX_REQUIRE(a && b && out_legs, TNX_ERR_ARG, "null argument");
+  TNX_REQUIRE(n_sign >= 0 && (n_sign == 0 || sign_pairs), TNX_ERR_ARG, "bad sign-pair list");
+  for (int t = 0; t < 2 * n_sign; ++t)
+    TNX_REQUIRE(sign_pairs[t] >= 0 && sign_pairs[t] < a->rank + b->rank, TNX_ERR_ARG,
+                "sign pair names a leg outside [A legs, B legs]");
   TNX_REQUIRE(nc >= 0 && nc <= a->rank && nc <= b->rank, TNX_ERR_ARG, "bad contracted leg count");
   const int rOut = a->rank + b->rank - 2 * nc;
   TNX_REQUIRE(rOut >= 1 && rOut <= MAX_RANK, TNX_ERR_ARG, "output rank out of range");
@@ -669,8 +690,10 @@ static void build_pairwise(PairwisePlan &P, const tnx_blocks *a, const tnx_block
   hipLaunchKernelGGL(segments_kernel, dim3(nblk(npairs)), dim3(256), 0, s, npairs, s_oid.p, pbeg.p, pend.p);
   P.prog.pairs.alloc(npairs);
   P.prog.outs.alloc(nout);
+  DevBuf<int32_t> sgn;
+  if (n_sign) sgn.upload(sign_pairs, 2 * (size_t)n_sign, s);
   hipLaunchKernelGGL(build_pairs_kernel, dim3(nblk(npairs)), dim3(256), 0, s, npairs, sp, a_kfirst, b_kfirst,
-                     s_idx.p, sA.p, sB.p, oA.p, oB.p, P.prog.pairs.p);
+                     s_idx.p, sA.p, sB.p, oA.p, oB.p, kA.p, kB.p, sgn.p, n_sign, P.prog.pairs.p);
   hipLaunchKernelGGL(build_outs_kernel, dim3(nblk(nout)), dim3(256), 0, s, nout, ooff.p, oM.p, oN.p, pbeg.p,
                      pend.p, P.prog.outs.p);
   TNX_CHECK_HIP(hipGetLastError());
@@ -754,11 +777,23 @@ __global__ void transpose_kernel(const TDesc *__restrict__ desc, const int32_t *
     r /= d.shape[k];
     src += ix * d.stride[k];
   }
-  out[d.out_off + e] = in[d.in_off + src];
+  const double v = in[d.in_off + src];
+  out[d.out_off + e] = d.neg ? -v : v;
+}
+
+// Koszul sign of a block under a leg permutation (tensor.py:32-50, 853-856):
+// -1 per inversion (i < j, perm[i] > perm[j]) whose two legs carry odd
+// charges (parity = charge & 1).
+static int koszul_negative(const int32_t *key, const int32_t *perm, int r) {
+  int neg = 0;
+  for (int i = 0; i < r; ++i)
+    for (int j = i + 1; j < r; ++j)
+      if (perm[i] > perm[j]) neg ^= (key[perm[i]] & 1) & (key[perm[j]] & 1);
+  return neg;
 }
 
 static void build_transpose(TransposePlan &P, const tnx_blocks *a, const tnx_leg *legs, const int32_t *perm,
-                            hipStream_t s) {
+                            hipStream_t s, bool graded = false) {
   TNX_REQUIRE(a && legs && perm, TNX_ERR_ARG, "null argument");
   const int r = a->rank, nb = a->n_blocks;
   TNX_REQUIRE(r >= 1 && r <= MAX_RANK, TNX_ERR_ARG, "rank out of range");
@@ -802,6 +837,7 @@ static void build_transpose(TransposePlan &P, const tnx_blocks *a, const tnx_leg
     const int i = (int)h_order[k];
     TDesc d{};
     d.rank = r;
+    d.neg = graded ? koszul_negative(a->keys + (int64_t)i * r, perm, r) : 0;
     d.in_off = a->offsets[i];
     d.out_off = off;
     int64_t st[MAX_RANK];
@@ -880,6 +916,19 @@ int tnx_plan_pairwise(const tnx_blocks *a, const tnx_blocks *b, int32_t a_kfirst
     std::unique_ptr<PairwisePlan> P(new PairwisePlan());
     build_pairwise(*P, a, b, a_kfirst, b_kfirst, n_contracted, out_legs, sym_n, mode, has_target != 0, target,
                    (hipStream_t)stream);
+    *plan = P.release();
+  });
+}
+
+int tnx_plan_pairwise_graded(const tnx_blocks *a, const tnx_blocks *b, int32_t a_kfirst, int32_t b_kfirst,
+                             int32_t n_contracted, const tnx_leg *out_legs, int32_t sym_n, int32_t mode,
+                             int32_t has_target, int32_t target, int32_t n_sign_pairs, const int32_t *sign_pairs,
+                             void *stream, tnx_plan **plan) {
+  return guarded([&] {
+    TNX_REQUIRE(plan, TNX_ERR_ARG, "null plan pointer");
+    std::unique_ptr<PairwisePlan> P(new PairwisePlan());
+    build_pairwise(*P, a, b, a_kfirst, b_kfirst, n_contracted, out_legs, sym_n, mode, has_target != 0, target,
+                   (hipStream_t)stream, n_sign_pairs, sign_pairs);
     *plan = P.release();
   });
 }
@@ -981,6 +1030,16 @@ int tnx_plan_transpose(const tnx_blocks *a, const tnx_leg *legs, const int32_t *
     TNX_REQUIRE(plan, TNX_ERR_ARG, "null plan pointer");
     std::unique_ptr<TransposePlan> P(new TransposePlan());
     build_transpose(*P, a, legs, perm, (hipStream_t)stream);
+    *plan = P.release();
+  });
+}
+
+int tnx_plan_transpose_graded(const tnx_blocks *a, const tnx_leg *legs, const int32_t *perm, int32_t graded,
+                              void *stream, tnx_plan **plan) {
+  return guarded([&] {
+    TNX_REQUIRE(plan, TNX_ERR_ARG, "null plan pointer");
+    std::unique_ptr<TransposePlan> P(new TransposePlan());
+    build_transpose(*P, a, legs, perm, (hipStream_t)stream, graded != 0);
     *plan = P.release();
   });
 }

@@ -290,6 +290,17 @@ def _keep_count(s_host, max_singular_values, max_truncation_err):
     return max(1, min(n_keep, n_total))
 
 
+def _koszul_negative(parities, perm) -> bool:
+    """True when the permutation has an odd number of inversions between legs
+    of odd parity (tensor.py:32-50)."""
+    n = 0
+    for i in range(len(perm)):
+        for j in range(i + 1, len(perm)):
+            if perm[i] > perm[j] and parities[perm[i]] and parities[perm[j]]:
+                n += 1
+    return n % 2 == 1
+
+
 def _assemble(t: SymmetricTensor, lax, rax):
     """Sector matrices of t in sector visit order.
 
@@ -301,6 +312,8 @@ def _assemble(t: SymmetricTensor, lax, rax):
     shaped by the left / right legs (linalg.py:268-309).
     """
     sectors = _sector_layout(t, lax, rax)
+    sym = t.indices[0].symmetry
+    perm = list(lax) + list(rax)
     q_list, mats = [], []
     for q, sec in sectors.items():
         R, C = sec["R"], sec["C"]
@@ -311,7 +324,11 @@ def _assemble(t: SymmetricTensor, lax, rax):
             nr = sec["rows"][lk][0]
             nc = sec["cols"][rk][0]
             r0, c0 = sec["row_off"][lk], sec["col_off"][rk]
-            M[r0:r0 + nr, c0:c0 + nc] = t.block(i).reshape(nr, nc)
+            blk = t.block(i).reshape(nr, nc)
+            if sym.is_fermionic and _koszul_negative(sym.parity(t.keys[i]), perm):
+                # Koszul sign of the reordering to (left, right) legs (linalg.py:171-186)
+                blk = -blk
+            M[r0:r0 + nr, c0:c0 + nc] = blk
         mats.append(M)
         q_list.append(q)
     if not mats:

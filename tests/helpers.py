@@ -5,12 +5,14 @@ import numpy as np
 from oracle import core as oc
 
 
+def _oracle_sym(sym):
+    if sym.is_fermionic:
+        return oc.FermionicU1() if sym.n_values() is None else oc.FermionParity()
+    return oc.U1() if sym.n_values() is None else oc.Zn(sym.n_values())
+
+
 def oracle_legs(indices):
-    out = []
-    for ix in indices:
-        sym = oc.U1() if ix.symmetry.n_values() is None else oc.Zn(ix.symmetry.n_values())
-        out.append(oc.Leg(sym, ix.charges, int(ix.flow), ix.label))
-    return tuple(out)
+    return tuple(oc.Leg(_oracle_sym(ix.symmetry), ix.charges, int(ix.flow), ix.label) for ix in indices)
 
 
 def to_oracle(t):
@@ -25,11 +27,14 @@ def to_oracle(t):
 
 
 def product_indices(legs):
-    from tenax_amd import FlowDirection, TensorIndex, U1Symmetry, ZnSymmetry
+    from tenax_amd import FermionicU1, FermionParity, FlowDirection, TensorIndex, U1Symmetry, ZnSymmetry
 
     out = []
     for l in legs:
-        sym = U1Symmetry() if l.sym.n is None else ZnSymmetry(l.sym.n)
+        if oc.is_fermionic(l.sym):
+            sym = FermionicU1() if l.sym.n is None else FermionParity()
+        else:
+            sym = U1Symmetry() if l.sym.n is None else ZnSymmetry(l.sym.n)
         out.append(TensorIndex(sym, l.charges, FlowDirection(l.flow), l.label))
     return tuple(out)
 
