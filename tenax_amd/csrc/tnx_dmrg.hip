@@ -239,30 +239,86 @@ __global__ void coef_kernel(int64_t n, const CoefSpec *__restrict__ cs, const do
     }
 }
 
-__global__ void mix_kernel(const MixItem *__restrict__ items, const int32_t *__restrict__ task_item,
-                           const int64_t *__restrict__ task_start, const MixTerm *__restrict__ terms,
-                           const double *__restrict__ coef, const double *__restrict__ X, double *__restrict__ Y) {
-  const MixItem it = items[task_item[blockIdx.x]];
-  const int64_t e = task_start[blockIdx.x] + threadIdx.x;
-  const int64_t size = (int64_t)it.D0 * it.D1 * it.D2;
-  if (e >= size) return;
-  const int i2 = (int)(e % it.D2);
-  const int64_t r = e / it.D2;
-  const int i1 = (int)(r % it.D1);
-  const int i0 = (int)(r / it.D1);
-  int j, u, v;
-  if (it.ca == 0) { j = i0; u = i1; v = i2; }
-  else if (it.ca == 1) { j = i1; u = i0; v = i2; }
-  else { j = i2; u = i0; v = i1; }
-  const int nj = it.ca == 0 ? it.D0 : (it.ca == 1 ? it.D1 : it.D2);
-  double acc = 0.0;
-  for (int t = it.term_begin; t < it.term_end; ++t) {
-    const MixTerm m = terms[t];
-    const double *xp = X + m.x_off + (int64_t)u * m.su + (int64_t)v * m.sv;
-    const double *cp = coef + m.coef_off + j;
-    for (int b = 0; b < m.nb; ++b) acc += cp[(int64_t)b * nj] * xp[(int64_t)b * m.sb];
+// Y[.., j, ..] = sum_terms sum_b coef[b, j] X_t[b, u, v] with j on axis ca.
+// One thread per (u, v) and chunk of MIX_NJ coefficient indices j, so each X
+// element is loaded once per term and reused for every j of the chunk.  The
+// (u, v) range of an item is padded to whole workgroups per chunk, so the
+// chunk (and every coefficient address) is uniform within a workgroup.
+constexpr int MIX_NJ = 4;
+
+__device__ __forceinline__ int mix_nj(int D0, int D1, int D2, int ca) {
+  return ca == 0 ? D0 : (ca == 1 ? D1 : D2);
+}
+
+// Item and term fields are read one by one (uniform across the workgroup, so
+// they go through scalar loads); a copied struct whose field is picked by a
+// run-time axis ends up in LDS.
+__global__ void __launch_bounds__(256) mix_kernel(const MixItem *__restrict__ items,
+                                                  const int32_t *__restrict__ task_item,
+                                                  const int64_t *__restrict__ task_start,
+                                                  const MixTerm *__restrict__ terms, const double *__restrict__ coef,
+                                                  const double *__restrict__ X, double *__restrict__ Y) {
+  const MixItem *ip = items + task_item[blockIdx.x];
+  const int D0 = ip->D0, D1 = ip->D1, D2 = ip->D2, ca = ip->ca;
+  const int nj = mix_nj(D0, D1, D2, ca);
+  const int64_t nuv = (int64_t)D0 * D1 * D2 / nj;
+  const int64_t nuvp = (nuv + 255) / 256 * 256;
+  const int64_t start = task_start[blockIdx.x];
+  const int jc = (int)(start / nuvp);
+  const int j0 = jc * MIX_NJ;
+  const int64_t uv = start - (int64_t)jc * nuvp + threadIdx.x;
+  if (uv >= nuv) return;
+  // (u, v) = the two non-coefficient indices in storage order
+  const int dv = ca == 2 ? D1 : D2;
+  int u, v;
+  if (nuv < (int64_t(1) << 31)) {
+    u = (int)((uint32_t)uv / (uint32_t)dv);
+    v = (int)uv - u * dv;
+  } else {
+    u = (int)(uv / dv);
+    v = (int)(uv - (int64_t)u * dv);
   }
-  Y[it.y_off + e] = acc;
+  // four named accumulators: a private array would be placed in LDS
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  const int nq = min(MIX_NJ, nj - j0);
+  const int t1 = ip->term_end;
+  for (int t = ip->term_begin; t < t1; ++t) {
+    const MixTerm *mp = terms + t;
+    const int64_t sb = mp->sb;
+    const int nb = mp->nb;
+    const double *xp = X + mp->x_off + (int64_t)u * mp->su + (int64_t)v * mp->sv;
+    const double *cp = coef + mp->coef_off + j0;
+    // up to four X loads in flight before their products
+    for (int b = 0; b < nb; b += 4) {
+      const int nbb = min(4, nb - b);
+      const double *xb = xp + (int64_t)b * sb;
+      double x[4];
+      x[0] = xb[0];
+      x[1] = nbb > 1 ? xb[sb] : 0.0;
+      x[2] = nbb > 2 ? xb[2 * sb] : 0.0;
+      x[3] = nbb > 3 ? xb[3 * sb] : 0.0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (r < nbb) {
+          const double *c = cp + (int64_t)(b + r) * nj;
+          a0 += c[0] * x[r];
+          if (nq > 1) a1 += c[1] * x[r];
+          if (nq > 2) a2 += c[2] * x[r];
+          if (nq > 3) a3 += c[3] * x[r];
+        }
+      }
+    }
+  }
+  // element offset of (j, u, v) in the item's row-major (D0, D1, D2) block
+  int64_t base, sj;
+  if (ca == 0) { base = uv; sj = nuv; }
+  else if (ca == 1) { base = (int64_t)u * D1 * D2 + v; sj = D2; }
+  else { base = uv * D2; sj = 1; }
+  double *yp = Y + ip->y_off + base + (int64_t)j0 * sj;
+  yp[0] = a0;
+  if (nq > 1) yp[sj] = a1;
+  if (nq > 2) yp[2 * sj] = a2;
+  if (nq > 3) yp[3 * sj] = a3;
 }
 
 // ------------------------------------------------------------------ plan-building kernels
@@ -510,8 +566,11 @@ __global__ void stage2_outs_kernel(int64_t nout, int kind, TView OutT, const int
 __global__ void mix_task_count_kernel(int64_t ni, const MixItem *__restrict__ items, int64_t *__restrict__ cnt) {
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (k >= ni) return;
-  const int64_t sz = (int64_t)items[k].D0 * items[k].D1 * items[k].D2;
-  cnt[k] = (sz + 255) / 256;
+  const MixItem *ip = items + k;
+  const int nj = mix_nj(ip->D0, ip->D1, ip->D2, ip->ca);
+  const int64_t nuv = nj ? (int64_t)ip->D0 * ip->D1 * ip->D2 / nj : 0;
+  // workgroups of 256 (u, v) pairs, per chunk of MIX_NJ coefficient indices
+  cnt[k] = (nj + MIX_NJ - 1) / MIX_NJ * ((nuv + 255) / 256);
 }
 
 __global__ void mix_task_write_kernel(int64_t ni, const int64_t *__restrict__ cnt, const int64_t *__restrict__ off,
