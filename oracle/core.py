@@ -191,14 +191,12 @@ def block_slices(legs, key):
 
 # ------------------------------------------------------------ generator
 def normal(seed_words, shape):
-    """Documented initial-state generator shared by the oracle and the GPU path.
+    """N(0,1) block for the key PRNGKey(seed_words[0]) folded with the later
+    words (jax.random.PRNGKey/fold_in/normal as used by core/tensor.py:655-660
+    and dmrg.py:1484): the Threefry-2x32 stream of oracle/random.py."""
+    from oracle import random as orand
 
-    N(0,1) from NumPy's PCG64 seeded with SeedSequence(seed_words).  Stands in
-    for jax.random.fold_in/normal (core/tensor.py:655-660), which cannot be
-    reproduced without JAX.
-    """
-    g = np.random.Generator(np.random.PCG64(np.random.SeedSequence(list(seed_words))))
-    return g.standard_normal(shape)
+    return orand.normal(orand.key_from_words(tuple(seed_words)), tuple(shape))
 
 
 # ------------------------------------------------------ symmetric tensor

@@ -31,19 +31,36 @@ def _device():
 
 
 # ------------------------------------------------------------------ generator
+# jax.random's 64-bit word layout: 1 = "partitionable" (jax's default since
+# 0.5), 0 = the original layout of earlier jax versions.
+RANDOM_LAYOUT = 1
+
+
 class PRNGKey:
-    """Seed words for the documented initial-state generator.
+    """Seed words of the initial-state generator.
 
     ``PRNGKey(seed)`` and ``fold_in(key, i)`` mirror jax.random.PRNGKey/fold_in
-    as used by SymmetricTensor.random_normal (tensor.py:655-660); values come
-    from NumPy PCG64 seeded with SeedSequence(words), N(0,1) by its standard
-    normal sampler.  JAX's threefry stream cannot be reproduced without JAX.
+    as the reference uses them (tensor.py:655-660, dmrg.py:1484): the words
+    (seed, i1, i2, ...) stand for PRNGKey(seed) folded with i1, i2, ...; the
+    Threefry-2x32 key is derived by libtnx (``key``), and the normals are
+    drawn on the device (``tnx_random_normal_blocks``).
     """
 
     __slots__ = ("words",)
 
     def __init__(self, seed):
         self.words = tuple(int(w) for w in (seed if isinstance(seed, (tuple, list)) else (seed,)))
+
+    @property
+    def key(self) -> tuple[int, int]:
+        lib = _lib.load(require_gpu=False)
+        s = self.words[0]
+        k = ((s >> 32) & 0xFFFFFFFF, s & 0xFFFFFFFF)
+        out = (_lib.C.c_uint32 * 2)()
+        for w in self.words[1:]:
+            _lib.check(lib.tnx_threefry2x32(k[0], k[1], 0, int(w) & 0xFFFFFFFF, out))
+            k = (int(out[0]), int(out[1]))
+        return k
 
     def __repr__(self):
         return f"PRNGKey{self.words}"
@@ -53,9 +70,25 @@ def fold_in(key: PRNGKey, i: int) -> PRNGKey:
     return PRNGKey(key.words + (int(i),))
 
 
+def _draw_blocks(keys, sizes, stddev: float = 1.0) -> torch.Tensor:
+    """Packed N(0, stddev) blocks, block b from key ``keys[b]`` (device)."""
+    lib = _lib.load()
+    sizes = np.asarray(sizes, np.int64)
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64) if len(sizes) else sizes
+    out = torch.empty(int(sizes.sum()), dtype=torch.float64, device=_device())
+    if len(sizes) and out.numel():
+        kk = np.ascontiguousarray(np.asarray([k.key for k in keys], np.uint32).reshape(-1))
+        _lib.check(lib.tnx_random_normal_blocks(len(sizes), kk.ctypes.data_as(_lib.C.POINTER(_lib.C.c_uint32)),
+                                                _lib.i64p(offs), _lib.i64p(sizes), RANDOM_LAYOUT, float(stddev),
+                                                _lib.dptr(out), _lib.stream_handle()))
+    return out
+
+
 def normal(key: PRNGKey, shape) -> np.ndarray:
-    g = np.random.Generator(np.random.PCG64(np.random.SeedSequence(list(key.words))))
-    return g.standard_normal(shape)
+    """jax.random.normal(key, shape, float64) drawn on the device (host copy)."""
+    key = key if isinstance(key, PRNGKey) else PRNGKey(key)
+    n = int(np.prod(shape)) if len(shape) else 1
+    return _draw_blocks([key], [n]).cpu().numpy().reshape(shape)
 
 
 # ------------------------------------------------------------------ sectors
@@ -265,11 +298,8 @@ class SymmetricTensor(Tensor):
         key = key if isinstance(key, PRNGKey) else PRNGKey(key)
         keys, shapes = valid_blocks(tuple(indices), target)
         offs, tot = cls._layout(shapes)
-        host = np.empty(tot, np.float64)
-        for i in range(len(keys)):
-            n = int(np.prod(shapes[i]))
-            host[offs[i]: offs[i] + n] = normal(fold_in(key, i), tuple(shapes[i])).reshape(-1) * stddev
-        data = torch.from_numpy(host).to(_device())
+        sizes = np.prod(shapes, axis=1).astype(np.int64) if len(keys) else np.zeros(0, np.int64)
+        data = _draw_blocks([fold_in(key, i) for i in range(len(keys))], sizes, stddev)
         return cls._wrap(indices, keys, shapes, offs, data)
 
     @classmethod

@@ -222,5 +222,7 @@ def test_zeros_and_random_normal_structure(gpu):
     o = oc.SymT.random_normal(legs, (1,))
     assert_same_structure(Z, o)
     assert_same_structure(R, o)
-    assert max_rel_err(R, o) == 0.0
+    # identical Threefry words (tests/test_gpu_random.py); the normals differ
+    # only through the device erfinv vs scipy's, within a few ulp
+    assert max_rel_err(R, o) < 4 * np.finfo(float).eps
     assert float(Z.norm()) == 0.0
