@@ -5,8 +5,10 @@ L.Wl.Wr.R . theta at the central bond (sites 49, 50) of a U(1) MPS with the
 charge profile of a converged chi=2048 state (tenax_amd.workloads), FP64 MFMA.
 Reported alongside: the plain per-block GPU path on the same problem, the FP32
 and split-FP32 variants with their error, one full 50-step device Lanczos
-solve, the dominant kernel's roofline, and the oracle's per-combination CPU
-matvec (the reference's loop structure) on the same problem.
+solve, the dominant kernel's roofline, the oracle's per-combination CPU
+matvec (the reference's loop structure) on the same problem, and one full
+left-to-right DMRG sweep over the chi = 2048 chain (phases timed, two centre
+updates also on the per-block path).
 
 With ``--workload hotrg`` it instead runs the batched Z2 HOTRG workload: 64
 2D-Ising temperatures T = 2.0 + 0.6 k/63, chi = 48, 30 coarse-graining steps,
@@ -15,7 +17,7 @@ the iPEPS CTMRG workload: Heisenberg gate, D = 4, d = 2, chi = 64, QR
 projectors, site tensor N(0, 1) from seed 3: CTM sweeps on the dense and the
 block-sparse path, the 100-sweep AD forward and its implicit backward.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--sweep]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-sweep]
                        [--workload dmrg|hotrg]
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -424,7 +426,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--perblock-steps", type=int, default=2)
-    ap.add_argument("--sweep", action="store_true", help="also time one full DMRG sweep on the chi state")
+    ap.add_argument("--no-sweep", dest="sweep", action="store_false",
+                    help="skip timing one full DMRG sweep on the chi state (timed by default)")
     ap.add_argument("--workload", choices=("dmrg", "hotrg", "ctm"), default="dmrg")
     ap.add_argument("--matvec-only", action="store_true", help="fused matvec and stage timing only (profiling)")
     ap.add_argument("--calibrate-copy", action="store_true", help="run a 512 MiB device copy (counter calibration)")
