@@ -935,7 +935,9 @@ static void run_fused_perblock(FusedPlan &P, const double *s1A, const double *s1
 }
 
 // ------------------------------------------------------------------ Lanczos on the device
-constexpr int LZ_G = 256;  // reduction workgroups
+// reduction workgroups: enough loads in flight to stream a vector from HBM
+// (256 left the dot products latency-bound at ~1.5 TB/s)
+constexpr int LZ_G = 1024;
 
 struct LzState {
   int32_t stopped;
@@ -1048,6 +1050,32 @@ __global__ void lz_axpy_kernel(double *__restrict__ w, const double *__restrict_
     if (k > 0) t = t - vp[i] * b;
     w[i] = t;
   }
+}
+
+// The same update fused with the partial sums of w.w per workgroup (the
+// norm that follows when there is no re-orthogonalisation).
+__global__ void lz_axpy_norm_kernel(double *__restrict__ w, const double *__restrict__ v,
+                                    const double *__restrict__ vp, int64_t n, int k,
+                                    const double *__restrict__ alpha, const double *__restrict__ beta,
+                                    double *__restrict__ part, const LzState *__restrict__ st) {
+  if (st->stopped) return;
+  __shared__ double sh[256];
+  const double a = alpha[k];
+  const double b = k > 0 ? beta[k] : 0.0;
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double t = w[i] - v[i] * a;
+    if (k > 0) t = t - vp[i] * b;
+    w[i] = t;
+    acc += t * t;
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
 // w = w - sum_j V[j]*c[j], applied sequentially in j
@@ -1216,13 +1244,15 @@ static void lanczos_run(const FusedPlan &P, const double *L, const double *Wl, c
     run_fused(P, L, v, Wl, Wr, R, w, compute, s);
     hipLaunchKernelGGL(dot_partial_kernel, rg, vb, 0, s, v, w, n, part, W.st.p);
     hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), vb, 0, s, part, LZ_G, 1, k, tol, alpha, beta, W.st.p);
-    hipLaunchKernelGGL(lz_axpy_kernel, vg, vb, 0, s, w, v, vp, n, k, alpha, beta, W.st.p);
     if (reorth) {
+      hipLaunchKernelGGL(lz_axpy_kernel, vg, vb, 0, s, w, v, vp, n, k, alpha, beta, W.st.p);
       hipLaunchKernelGGL(dot_multi_partial_kernel, dim3(LZ_G, k + 1), vb, 0, s, V, n, k + 1, w, part, W.st.p);
       hipLaunchKernelGGL(lz_multi_finalize_kernel, dim3(k + 1), vb, 0, s, part, LZ_G, rc, W.st.p);
       hipLaunchKernelGGL(lz_reorth_kernel, vg, vb, 0, s, w, V, n, k + 1, rc, W.st.p);
+      hipLaunchKernelGGL(dot_partial_kernel, rg, vb, 0, s, w, w, n, part, W.st.p);
+    } else {
+      hipLaunchKernelGGL(lz_axpy_norm_kernel, rg, vb, 0, s, w, v, vp, n, k, alpha, beta, part, W.st.p);
     }
-    hipLaunchKernelGGL(dot_partial_kernel, rg, vb, 0, s, w, w, n, part, W.st.p);
     hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), vb, 0, s, part, LZ_G, 2, k, tol, alpha, beta, W.st.p);
     hipLaunchKernelGGL(lz_scale_kernel, vg, vb, 0, s, w, w, n, W.st.p, 1);
   }
