@@ -30,18 +30,29 @@ struct Stage {
   static constexpr int PB = BN / 64;
   double a[PA][4];
   double b[PB][4];
-  double sgn;  // -1 for a pair with a fermionic exchange sign, applied to A
+  uint32_t neg_hi;  // 0x80000000 for a pair with a fermionic exchange sign (flips A's sign bit)
 };
 
-// Operand loads go through a buffer resource spanning the one block: an
-// out-of-range element gets the offset `nbytes` and the hardware returns 0.
-// The loads carry no branches or selects, so the compiler keeps them in
-// flight across the MFMAs of the previous tile.  (Blocks are < 2^31 bytes,
-// checked when the program is finalized.)
+// A with the pair's sign: an XOR of the high word's sign bit (no FP64 multiply).
+__device__ __forceinline__ double signed_a(double x, uint32_t neg_hi) {
+  return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, x) ^ ((uint64_t)neg_hi << 32));
+}
+
+// Operand loads go through a buffer resource spanning exactly one block, and
+// the hardware returns 0 for any offset at or past its end.  That bounds K
+// without per-element tests: B rows k >= K (ldb >= N) and KM-layout A rows
+// k >= K (lda >= M) lie past the block's end.  Columns of B past N and rows
+// of A past M that still fall inside the block read other elements of it;
+// they only reach accumulator columns / rows outside the output block, which
+// are never stored.  MK-layout A still masks k >= K within a row (those
+// elements meet B's zero rows, but a non-finite value must not leak into the
+// sum).  The loads carry no branches, so the compiler keeps them in flight
+// across the MFMAs of the previous tile.  Offsets are 32-bit: finalize_program
+// checks every pair's extent plus one tile of overhang against 2^32 bytes.
 constexpr int32_t BUF_DWORD3 = 0x00020000;  // raw dword access, gfx9 family
 
-__device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t r, int32_t off) {
-  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+__device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int32_t)off, 0, 0));
 }
 
 // A operand: MK = a[m*lda+k] (4 threads per row of 16 k, 4 consecutive k
@@ -53,27 +64,25 @@ __device__ __forceinline__ void load_a(const double *__restrict__ A, const GemmP
   const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void *)(A + p.a_off), (short)0, nb, BUF_DWORD3);
 #pragma unroll
   for (int q = 0; q < BM / 64; ++q) {
-    // the run's start offset is computed unconditionally; each element then
-    // only selects between it (+8i) and the out-of-range offset
     if (ALAY == LAY_MK) {
-      const int m = m0 + (tid >> 2) + 64 * q;
+      // rows past M lie past the block's end; k >= K selects the end offset
+      const uint32_t m = (uint32_t)(m0 + (tid >> 2) + 64 * q);
       const int kq = k0 + ((tid & 3) << 2);
-      const int run = 8 * (m * p.lda + kq);
-      const int lim = m < M ? p.K - kq : 0;
+      const uint32_t run = 8u * (m * (uint32_t)p.lda + (uint32_t)kq);
+      const int lim = p.K - kq;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) r[q][i] = buf_ld(rs, i < lim ? run + 8 * i : nb);
+      for (int i = 0; i < 4; ++i) r[q][i] = buf_ld(rs, i < lim ? run + 8u * i : (uint32_t)nb);
     } else {
       // one element per thread per k row, consecutive lanes on consecutive m
-      // (coalesced 512-byte rows, conflict-free LDS stores)
+      // (coalesced 512-byte rows, conflict-free LDS stores); rows k >= K lie
+      // past the block's end
       constexpr int RPP = 256 / BM;  // k rows per pass
-      const int m = m0 + tid % BM;
-      const int kb = k0 + tid / BM;
-      const bool min = m < M;
+      const uint32_t m = (uint32_t)(m0 + tid % BM);
+      const uint32_t kb = (uint32_t)(k0 + tid / BM);
+      const uint32_t base = 8u * (kb * (uint32_t)p.lda + m);
+      const uint32_t step = 8u * RPP * (uint32_t)p.lda;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int k = kb + RPP * (4 * q + i);
-        r[q][i] = buf_ld(rs, (min && k < p.K) ? 8 * (k * p.lda + m) : nb);
-      }
+      for (int i = 0; i < 4; ++i) r[q][i] = buf_ld(rs, base + step * (uint32_t)(4 * q + i));
     }
   }
 }
@@ -85,16 +94,15 @@ __device__ __forceinline__ void load_b(const double *__restrict__ B, const GemmP
   constexpr int RPP = 256 / BN;  // k rows per pass; one element per thread per row
   const int32_t nb = (int32_t)(8 * ((int64_t)(p.K - 1) * p.ldb + N));
   const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void *)(B + p.b_off), (short)0, nb, BUF_DWORD3);
-  const int n = n0 + tid % BN;
-  const int kb = k0 + tid / BN;
-  const bool nin = n < N;
+  // rows k >= K lie past the block's end; columns past N are never stored
+  const uint32_t n = (uint32_t)(n0 + tid % BN);
+  const uint32_t kb = (uint32_t)(k0 + tid / BN);
+  const uint32_t base = 8u * (kb * (uint32_t)p.ldb + n);
+  const uint32_t step = 8u * RPP * (uint32_t)p.ldb;
 #pragma unroll
   for (int q = 0; q < BN / 64; ++q)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int k = kb + RPP * (4 * q + i);
-      r[q][i] = buf_ld(rs, (nin && k < p.K) ? 8 * (k * p.ldb + n) : nb);
-    }
+    for (int i = 0; i < 4; ++i) r[q][i] = buf_ld(rs, base + step * (uint32_t)(4 * q + i));
 }
 
 // LDS positions ([k][m] and [k][n], pitch BM+1 / BN+1) of a thread's staged values.
@@ -118,7 +126,7 @@ __device__ __forceinline__ void store_tiles(T *As, T *Bs, const Stage<BM, BN> &s
   for (int q = 0; q < BM / 64; ++q)
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      As[a_lds<ALAY, BM>(tid, q, i)] = (T)(st.a[q][i] * st.sgn);
+      As[a_lds<ALAY, BM>(tid, q, i)] = (T)signed_a(st.a[q][i], st.neg_hi);
 #pragma unroll
   for (int q = 0; q < BN / 64; ++q)
 #pragma unroll
@@ -134,7 +142,7 @@ __device__ __forceinline__ void store_tiles_split(float *Ah, float *Al, float *B
   for (int q = 0; q < BM / 64; ++q)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const double x = st.a[q][i] * st.sgn;
+      const double x = signed_a(st.a[q][i], st.neg_hi);
       const float h = (float)x;
       const int at = a_lds<ALAY, BM>(tid, q, i);
       Ah[at] = h;
@@ -347,7 +355,7 @@ __global__ __launch_bounds__(GEMM_THREADS) __attribute__((amdgpu_waves_per_eu(
   auto load = [&](Stage<BM, BN> &st) {
     load_a<ALAY, BM>(A, p, o.M, t.m0, lk0, tid, st.a);
     load_b<BN>(B, p, o.N, t.n0, lk0, tid, st.b);
-    st.sgn = p.neg ? -1.0 : 1.0;
+    st.neg_hi = p.neg ? 0x80000000u : 0u;
     advance();
   };
 

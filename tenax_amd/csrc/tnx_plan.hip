@@ -405,6 +405,32 @@ __global__ void write_tasks_kernel(int64_t nout, const uint32_t *__restrict__ or
     for (int b = 0; b < tn; ++b) tasks[t++] = whole_tile(o, a * GEMM_BM, b * GEMM_BN);
 }
 
+// The grouped GEMM addresses each operand block through a 32-bit buffer
+// offset and relies on the block's end for its K bound: every pair needs
+// lda >= K (MK) or lda >= M (KM), ldb >= N, and its extent plus one tile of
+// overhang (the largest tile, 128 x 128 x 16) below 2^32 bytes.
+static void check_operand_extents(const GemmProgram &prog) {
+  constexpr int64_t LIM = int64_t(1) << 32;
+  constexpr int64_t OVER = 128;
+  for (const GemmOut &o : prog.h_outs)
+    for (int k = o.pair_begin; k < o.pair_end; ++k) {
+      const GemmPair &p = prog.h_pairs[k];
+      if (p.K <= 0) continue;  // no K tiles: never loaded
+      int64_t a_end, b_end;
+      if (prog.a_lay == LAY_MK) {
+        TNX_REQUIRE(p.lda >= p.K, TNX_ERR_ARG, "A block row pitch below K");
+        a_end = 8 * ((int64_t)(o.M + OVER) * p.lda + p.K + GEMM_BK);
+      } else {
+        TNX_REQUIRE(p.lda >= o.M, TNX_ERR_ARG, "A block row pitch below M");
+        a_end = 8 * ((int64_t)(p.K + GEMM_BK) * p.lda + o.M + OVER);
+      }
+      TNX_REQUIRE(p.ldb >= o.N, TNX_ERR_ARG, "B block row pitch below N");
+      b_end = 8 * ((int64_t)(p.K + GEMM_BK) * p.ldb + o.N + OVER);
+      TNX_REQUIRE(a_end < LIM && b_end < LIM, TNX_ERR_CAPACITY,
+                  "block too large for the grouped GEMM's 32-bit operand offsets");
+    }
+}
+
 void finalize_program(GemmProgram &prog, hipStream_t s) {
   const int64_t nout = prog.n_outs;
   prog.n_tasks = 0;
@@ -441,6 +467,7 @@ void finalize_program(GemmProgram &prog, hipStream_t s) {
   prog.outs.download(prog.h_outs.data(), prog.n_outs, s);
   prog.pairs.download(prog.h_pairs.data(), prog.n_pairs, s);
   TNX_CHECK_HIP(hipStreamSynchronize(s));
+  check_operand_extents(prog);
   double f = 0;
   for (const auto &o : prog.h_outs)
     for (int k = o.pair_begin; k < o.pair_end; ++k) f += 2.0 * o.M * (double)o.N * prog.h_pairs[k].K;
