@@ -849,15 +849,19 @@ static void build_fused(FusedPlan &P, int kind, DevTable &P1, DevTable &P2, DevT
   P.n_tasks = P.s1.n_tasks + P.s2.n_tasks + P.n_mix_tasks;
 }
 
+// coef_ready: the MPO coefficients of these W1/W2 are already in P.coef (the
+// Lanczos iteration applies one operator many times).
 static void run_fused(const FusedPlan &P, const double *s1A, const double *s1B, const double *W1, const double *W2,
-                      const double *s2B, double *out, int compute, hipStream_t s) {
+                      const double *s2B, double *out, int compute, hipStream_t s, bool coef_ready = false) {
   if (P.n_terms == 0) {
     if (P.out_total > 0)
       hipLaunchKernelGGL(zero_kernel, dim3(nb256(P.out_total)), dim3(256), 0, s, P.out_total, out);
     TNX_CHECK_HIP(hipGetLastError());
     return;
   }
-  hipLaunchKernelGGL(coef_kernel, dim3(nb256(P.n_terms)), dim3(256), 0, s, P.n_terms, P.cspec.p, W1, W2, P.coef.p);
+  if (!coef_ready)
+    hipLaunchKernelGGL(coef_kernel, dim3(nb256(P.n_terms)), dim3(256), 0, s, P.n_terms, P.cspec.p, W1, W2,
+                       P.coef.p);
   launch_grouped_gemm(P.s1, s1A, s1B, P.X.p, compute, false, s);
   hipLaunchKernelGGL(mix_kernel, dim3(P.n_mix_tasks), dim3(256), 0, s, P.items.p, P.mix_titem.p, P.mix_tstart.p,
                      P.mterms.p, P.coef.p, P.X.p, P.Y.p);
@@ -1241,7 +1245,7 @@ static void lanczos_run(const FusedPlan &P, const double *L, const double *Wl, c
     double *v = V + (int64_t)k * n;
     double *w = V + (int64_t)(k + 1) * n;
     const double *vp = k > 0 ? V + (int64_t)(k - 1) * n : v;
-    run_fused(P, L, v, Wl, Wr, R, w, compute, s);
+    run_fused(P, L, v, Wl, Wr, R, w, compute, s, k > 0);
     hipLaunchKernelGGL(dot_partial_kernel, rg, vb, 0, s, v, w, n, part, W.st.p);
     hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), vb, 0, s, part, LZ_G, 1, k, tol, alpha, beta, W.st.p);
     if (reorth) {
