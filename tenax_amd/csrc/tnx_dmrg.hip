@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <mutex>
+#include <unordered_map>
 
 #include "../../include/tnx_dmrg.h"
 
@@ -216,6 +217,15 @@ struct MixItem {
   int32_t term_begin, term_end;
 };
 
+// One (term, b) product of an item: X element at x_off + u*su + v*sv, times
+// coef[coef_off + j] (32 bytes: one scalar load per entry).
+struct MixEntry {
+  int64_t x_off;
+  int64_t coef_off;
+  int32_t su, sv;
+  int64_t pad;
+};
+
 struct CoefSpec {
   int64_t w1_off, w2_off, coef_off;
   int32_t kind, nb, nmid, nj;
@@ -250,15 +260,22 @@ __device__ __forceinline__ int mix_nj(int D0, int D1, int D2, int ca) {
   return ca == 0 ? D0 : (ca == 1 ? D1 : D2);
 }
 
-// Item and term fields are read one by one (uniform across the workgroup, so
-// they go through scalar loads); a copied struct whose field is picked by a
-// run-time axis ends up in LDS.
+// Item fields are read one by one (uniform across the workgroup, so they go
+// through scalar loads; a copied struct whose field is picked by a run-time
+// axis ends up in LDS).  The item's (term, b) products come as a flat entry
+// list, MIX_BATCH X loads are issued before their products, so a wave waits
+// on memory once per batch rather than once per term.
+constexpr int MIX_BATCH = 8;
+
 __global__ void __launch_bounds__(256) mix_kernel(const MixItem *__restrict__ items,
                                                   const int32_t *__restrict__ task_item,
                                                   const int64_t *__restrict__ task_start,
-                                                  const MixTerm *__restrict__ terms, const double *__restrict__ coef,
+                                                  const int32_t *__restrict__ item_entry,
+                                                  const MixEntry *__restrict__ ents, const double *__restrict__ coef,
                                                   const double *__restrict__ X, double *__restrict__ Y) {
-  const MixItem *ip = items + task_item[blockIdx.x];
+  const int32_t item = task_item[blockIdx.x];
+  if (item < 0) return;  // padding of the fused launch order
+  const MixItem *ip = items + item;
   const int D0 = ip->D0, D1 = ip->D1, D2 = ip->D2, ca = ip->ca;
   const int nj = mix_nj(D0, D1, D2, ca);
   const int64_t nuv = (int64_t)D0 * D1 * D2 / nj;
@@ -281,33 +298,24 @@ __global__ void __launch_bounds__(256) mix_kernel(const MixItem *__restrict__ it
   // four named accumulators: a private array would be placed in LDS
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
   const int nq = min(MIX_NJ, nj - j0);
-  const int t1 = ip->term_end;
-  for (int t = ip->term_begin; t < t1; ++t) {
-    const MixTerm *mp = terms + t;
-    const int64_t sb = mp->sb;
-    const int nb = mp->nb;
-    const double *xp = X + mp->x_off + (int64_t)u * mp->su + (int64_t)v * mp->sv;
-    const double *cp = coef + mp->coef_off + j0;
-    // up to four X loads in flight before their products
-    for (int b = 0; b < nb; b += 4) {
-      const int nbb = min(4, nb - b);
-      const double *xb = xp + (int64_t)b * sb;
-      double x[4];
-      x[0] = xb[0];
-      x[1] = nbb > 1 ? xb[sb] : 0.0;
-      x[2] = nbb > 2 ? xb[2 * sb] : 0.0;
-      x[3] = nbb > 3 ? xb[3 * sb] : 0.0;
+  const int e1 = item_entry[item + 1];
+  for (int e = item_entry[item]; e < e1; e += MIX_BATCH) {
+    double x[MIX_BATCH];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        if (r < nbb) {
-          const double *c = cp + (int64_t)(b + r) * nj;
-          a0 += c[0] * x[r];
-          if (nq > 1) a1 += c[1] * x[r];
-          if (nq > 2) a2 += c[2] * x[r];
-          if (nq > 3) a3 += c[3] * x[r];
-        }
+    for (int k = 0; k < MIX_BATCH; ++k)
+      if (e + k < e1) {
+        const MixEntry *mp = ents + e + k;
+        x[k] = X[mp->x_off + (int64_t)u * mp->su + (int64_t)v * mp->sv];
       }
-    }
+#pragma unroll
+    for (int k = 0; k < MIX_BATCH; ++k)
+      if (e + k < e1) {
+        const double *c = coef + ents[e + k].coef_off + j0;
+        a0 += c[0] * x[k];
+        if (nq > 1) a1 += c[1] * x[k];
+        if (nq > 2) a2 += c[2] * x[k];
+        if (nq > 3) a3 += c[3] * x[k];
+      }
   }
   // element offset of (j, u, v) in the item's row-major (D0, D1, D2) block
   int64_t base, sj;
@@ -595,9 +603,14 @@ struct FusedPlan : public tnx_plan_s {
   DevBuf<MixItem> items;
   DevBuf<MixTerm> mterms;
   DevBuf<CoefSpec> cspec;
-  DevBuf<int32_t> mix_titem;
+  DevBuf<int32_t> mix_titem;   // item-contiguous tasks (per-block path)
   DevBuf<int64_t> mix_tstart;
   int32_t n_mix_tasks = 0;
+  DevBuf<int32_t> item_entry;  // per item: its range of (term, b) entries
+  DevBuf<MixEntry> ents;
+  DevBuf<int32_t> mixf_titem;  // fused launch order (items sharing X blocks together, per XCD)
+  DevBuf<int64_t> mixf_tstart;
+  int32_t n_mixf_tasks = 0;
   int64_t n_terms = 0, n_items = 0;
   DevBuf<double> X, Y, coef;
   int64_t x_total = 0, y_total = 0, coef_total = 0;
@@ -614,6 +627,102 @@ struct FusedPlan : public tnx_plan_s {
   std::vector<uint8_t> pb2_first;
   std::vector<GemmOut> pb2_zero;  // outs without pairs: zero-filled
 };
+
+// Launch order of the fused mix.  An X block (one stage-1 pair) feeds several
+// items (one per output physical configuration), so each X element is read
+// once per item.  Items linked by shared X blocks (union-find over the terms'
+// X offsets) have the same (u, v) extent; their workgroups for one (u, v)
+// slice are placed next to each other, and the whole list is cut into N_XCD
+// contiguous bins dealt round robin (workgroup w runs on XCD w % N_XCD), so
+// the items' reads of one X slice meet in one L2 at about the same time.
+// Padding entries carry item -1.
+static void order_mix_tasks(FusedPlan &P, int64_t nI, hipStream_t s) {
+  std::vector<MixItem> items(nI);
+  std::vector<MixTerm> terms(P.n_terms);
+  P.items.download(items.data(), nI, s);
+  if (P.n_terms) P.mterms.download(terms.data(), P.n_terms, s);
+  TNX_CHECK_HIP(hipStreamSynchronize(s));
+  // flat (term, b) entries per item
+  {
+    std::vector<int32_t> ie(nI + 1, 0);
+    std::vector<MixEntry> en;
+    for (int64_t i = 0; i < nI; ++i) {
+      const MixItem &it = items[i];
+      const int nj = it.ca == 0 ? it.D0 : (it.ca == 1 ? it.D1 : it.D2);
+      for (int32_t t = it.term_begin; t < it.term_end; ++t) {
+        const MixTerm &m = terms[t];
+        TNX_REQUIRE(m.su < (int64_t(1) << 31) && m.sv < (int64_t(1) << 31), TNX_ERR_CAPACITY,
+                    "mix stride beyond 32 bits");
+        for (int b = 0; b < m.nb; ++b)
+          en.push_back(MixEntry{m.x_off + (int64_t)b * m.sb, m.coef_off + (int64_t)b * nj, (int32_t)m.su,
+                                (int32_t)m.sv, 0});
+      }
+      TNX_REQUIRE(en.size() < (size_t(1) << 31), TNX_ERR_CAPACITY, "too many mix entries");
+      ie[i + 1] = (int32_t)en.size();
+    }
+    if (en.empty()) en.push_back(MixEntry{0, 0, 0, 0, 0});
+    P.item_entry.upload(ie.data(), ie.size(), s);
+    P.ents.upload(en.data(), en.size(), s);
+  }
+  std::vector<int64_t> parent(nI);
+  for (int64_t i = 0; i < nI; ++i) parent[i] = i;
+  auto find = [&](int64_t x) {
+    while (parent[x] != x) x = parent[x] = parent[parent[x]];
+    return x;
+  };
+  auto nuv_of = [](const MixItem &it) {
+    const int nj = it.ca == 0 ? it.D0 : (it.ca == 1 ? it.D1 : it.D2);
+    return nj ? (int64_t)it.D0 * it.D1 * it.D2 / nj : 0;
+  };
+  std::unordered_map<int64_t, int64_t> first;  // X offset -> an item reading it
+  for (int64_t i = 0; i < nI; ++i)
+    for (int32_t t = items[i].term_begin; t < items[i].term_end; ++t) {
+      auto r = first.emplace(terms[t].x_off, i);
+      if (!r.second) {
+        const int64_t a = find(r.first->second), b = find(i);
+        if (a != b && nuv_of(items[a]) == nuv_of(items[b])) parent[b] = a;
+      }
+    }
+  std::vector<std::vector<int64_t>> groups;
+  std::unordered_map<int64_t, size_t> gid;
+  for (int64_t i = 0; i < nI; ++i) {
+    auto r = gid.emplace(find(i), groups.size());
+    if (r.second) groups.emplace_back();
+    groups[r.first->second].push_back(i);
+  }
+  std::vector<int32_t> ti;
+  std::vector<int64_t> ts;
+  ti.reserve(P.n_mix_tasks);
+  ts.reserve(P.n_mix_tasks);
+  for (const auto &g : groups) {
+    const int64_t nuv = nuv_of(items[g[0]]);
+    const int64_t per = (nuv + 255) / 256, nuvp = per * 256;
+    for (int64_t sl = 0; sl < per; ++sl)
+      for (int64_t i : g) {
+        const MixItem &it = items[i];
+        const int nj = it.ca == 0 ? it.D0 : (it.ca == 1 ? it.D1 : it.D2);
+        if (nj == 0 || nuv_of(it) == 0) continue;
+        for (int jc = 0; jc < (nj + MIX_NJ - 1) / MIX_NJ; ++jc) {
+          ti.push_back((int32_t)i);
+          ts.push_back((int64_t)jc * nuvp + sl * 256);
+        }
+      }
+  }
+  TNX_REQUIRE((int64_t)ti.size() == (int64_t)P.n_mix_tasks, TNX_ERR_BAD_STRUCTURE, "mix task count mismatch");
+  const int64_t n = (int64_t)ti.size();
+  const int64_t rows = (n + N_XCD - 1) / N_XCD;
+  std::vector<int32_t> fi(rows * N_XCD, -1);
+  std::vector<int64_t> fs(rows * N_XCD, 0);
+  for (int64_t p = 0; p < n; ++p) {
+    const int64_t b = p / rows, r = p % rows;  // bin b = XCD b, r-th task of the bin
+    fi[r * N_XCD + b] = ti[p];
+    fs[r * N_XCD + b] = ts[p];
+  }
+  P.mixf_titem.upload(fi.data(), fi.size(), s);
+  P.mixf_tstart.upload(fs.data(), fs.size(), s);
+  P.n_mixf_tasks = (int32_t)fi.size();
+  TNX_CHECK_HIP(hipStreamSynchronize(s));
+}
 
 static void build_fused(FusedPlan &P, int kind, DevTable &P1, DevTable &P2, DevTable &W1, DevTable *W2,
                         DevTable &Part, DevTable *OutT, const tnx_leg *out_legs, hipStream_t s) {
@@ -828,6 +937,7 @@ static void build_fused(FusedPlan &P, int kind, DevTable &P1, DevTable &P2, DevT
       P.pbm_off[k] = (int32_t)hmo[k];
       P.pbm_cnt[k] = (int32_t)hmc[k];
     }
+    order_mix_tasks(P, nI, s);
   }
   P.X.alloc(std::max<int64_t>(P.x_total, 1));
   P.Y.alloc(std::max<int64_t>(P.y_total, 1));
@@ -863,8 +973,8 @@ static void run_fused(const FusedPlan &P, const double *s1A, const double *s1B, 
     hipLaunchKernelGGL(coef_kernel, dim3(nb256(P.n_terms)), dim3(256), 0, s, P.n_terms, P.cspec.p, W1, W2,
                        P.coef.p);
   launch_grouped_gemm(P.s1, s1A, s1B, P.X.p, compute, false, s);
-  hipLaunchKernelGGL(mix_kernel, dim3(P.n_mix_tasks), dim3(256), 0, s, P.items.p, P.mix_titem.p, P.mix_tstart.p,
-                     P.mterms.p, P.coef.p, P.X.p, P.Y.p);
+  hipLaunchKernelGGL(mix_kernel, dim3(P.n_mixf_tasks), dim3(256), 0, s, P.items.p, P.mixf_titem.p, P.mixf_tstart.p,
+                     P.item_entry.p, P.ents.p, P.coef.p, P.X.p, P.Y.p);
   launch_grouped_gemm(P.s2, P.Y.p, s2B, out, compute, false, s);
   TNX_CHECK_HIP(hipGetLastError());
 }
@@ -927,7 +1037,7 @@ static void run_fused_perblock(FusedPlan &P, const double *s1A, const double *s1
                     P.s1.pairs.p, s1A, s1B, P.X.p, false);
   for (int64_t k = 0; k < P.n_items; ++k)
     hipLaunchKernelGGL(mix_kernel, dim3(P.pbm_cnt[k]), dim3(256), 0, s, P.items.p, P.mix_titem.p + P.pbm_off[k],
-                       P.mix_tstart.p + P.pbm_off[k], P.mterms.p, P.coef.p, P.X.p, P.Y.p);
+                       P.mix_tstart.p + P.pbm_off[k], P.item_entry.p, P.ents.p, P.coef.p, P.X.p, P.Y.p);
   for (const auto &z : P.pb2_zero) {
     const int64_t n = (int64_t)z.M * z.N;
     if (n) hipLaunchKernelGGL(zero_kernel, dim3(nb256(n)), dim3(256), 0, s, n, out + z.c_off);
@@ -1351,8 +1461,8 @@ int tnx_matvec_execute_stage(const tnx_plan *plan, int32_t stage, const double *
         launch_grouped_gemm(P.s1, L, x, P.X.p, compute, false, s, !gemm_only);
         break;
       case 2:
-        hipLaunchKernelGGL(mix_kernel, dim3(P.n_mix_tasks), dim3(256), 0, s, P.items.p, P.mix_titem.p,
-                           P.mix_tstart.p, P.mterms.p, P.coef.p, P.X.p, P.Y.p);
+        hipLaunchKernelGGL(mix_kernel, dim3(P.n_mixf_tasks), dim3(256), 0, s, P.items.p, P.mixf_titem.p,
+                           P.mixf_tstart.p, P.item_entry.p, P.ents.p, P.coef.p, P.X.p, P.Y.p);
         break;
       case 3:
         launch_grouped_gemm(P.s2, P.Y.p, R, y, compute, false, s, !gemm_only);
