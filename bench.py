@@ -516,6 +516,25 @@ def main():
     prob.x = x0
     log(f"lanczos 50 steps {lz_s:.3f}s E={e:.12f} n={n}")
 
+    # the same solve recorded once as a hipGraph, then replayed
+    torch.cuda.synchronize()
+    t0 = time.time()
+    graph = prob.lanczos_graph(50, 1e-12, False, 0)
+    torch.cuda.synchronize()
+    cap_s = time.time() - t0
+    xs = prob.x.clone()
+    graph.launch()
+    graph.result()
+    prob.x.copy_(xs)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    graph.launch()
+    eg, ng = graph.result()
+    lzg_s = time.time() - t0
+    graph.close()
+    prob.x.copy_(xs)
+    log(f"lanczos graph: capture+instantiate {cap_s:.3f}s, replay {lzg_s:.3f}s E={eg:.12f} (eager E diff {eg - e:.1e})")
+
     kname, kwgs, _ = stage_kernel(prob, dom)
     traffic = committed_traffic(kname, kwgs) if (args.L == 100 and args.chi == 2048) else None
     tiling = {f"stage{st}": stage_kernel(prob, st)[2] for st in (1, 3)}
@@ -549,6 +568,8 @@ def main():
         "per_block_gpu": {"ms": ms_pb, "gflops": flops / (ms_pb * 1e-3) / 1e9, "speedup_fused": ms_pb / ms},
         "reduced_precision": extra,
         "lanczos_50_steps_s": lz_s,
+        "lanczos_50_steps_graph": {"replay_s": lzg_s, "capture_instantiate_s": cap_s,
+                                   "energy_equal_to_eager": eg == e},
         "sweep_s": None,
     }
     if args.sweep:

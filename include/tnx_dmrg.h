@@ -85,6 +85,33 @@ int tnx_lanczos(const tnx_plan *matvec_plan, const double *L, const double *Wl, 
                 const double *R, double *theta, int32_t max_iter, double tol, int32_t reorth,
                 int32_t compute, double *energy, int32_t *n_steps, void *stream);
 
+/* Captured Lanczos solve (hipGraph).  No counterpart in the reference, whose
+ * Lanczos is a Python loop under JAX's asynchronous dispatch
+ * (src/tenax/algorithms/dmrg.py:978-1045): the whole device solve of
+ * tnx_lanczos -- every launch of its max_iter steps, the tridiagonal solve and
+ * the Ritz vector -- recorded once into an instantiated graph that owns its
+ * Krylov workspace, then replayed with one launch on any stream.  Results are
+ * bitwise those of tnx_lanczos with the same arguments.
+ *   create   records the solve for these operand and theta pointers;
+ *   update   re-records it for new pointers (or after the plan's tiling
+ *            changed) and updates the executable graph in place;
+ *   launch   replays it (TNX_ERR_ARG if tnx_plan_set_tiling changed the plan
+ *            since the last create/update);
+ *   result   synchronises the stream and returns energy and step count
+ *            (theta then holds the eigenvector);
+ *   free     releases the graph and its workspace.
+ * The plan and the operand buffers must outlive the graph. */
+typedef struct tnx_graph_s tnx_graph;
+int tnx_lanczos_graph_create(const tnx_plan *matvec_plan, const double *L, const double *Wl,
+                             const double *Wr, const double *R, double *theta, int32_t max_iter,
+                             double tol, int32_t reorth, int32_t compute, void *stream,
+                             tnx_graph **graph);
+int tnx_lanczos_graph_update(tnx_graph *graph, const double *L, const double *Wl, const double *Wr,
+                             const double *R, double *theta, void *stream);
+int tnx_graph_launch(tnx_graph *graph, void *stream);
+int tnx_lanczos_graph_result(tnx_graph *graph, void *stream, double *energy, int32_t *n_steps);
+int tnx_graph_free(tnx_graph *graph);
+
 #ifdef __cplusplus
 }
 #endif

@@ -99,6 +99,12 @@ SIGNATURES = {
                                C.POINTER(_p)]),
     "tnx_env_execute": (C.c_int, [_p, _p, _p, _p, _p, _i32, _p]),
     "tnx_lanczos": (C.c_int, [_p, _p, _p, _p, _p, _p, _i32, C.c_double, _i32, _i32, _PD, _PI32, _p]),
+    "tnx_lanczos_graph_create": (C.c_int, [_p, _p, _p, _p, _p, _p, _i32, C.c_double, _i32, _i32, _p,
+                                           C.POINTER(_p)]),
+    "tnx_lanczos_graph_update": (C.c_int, [_p, _p, _p, _p, _p, _p, _p]),
+    "tnx_graph_launch": (C.c_int, [_p, _p]),
+    "tnx_lanczos_graph_result": (C.c_int, [_p, _p, _PD, _PI32]),
+    "tnx_graph_free": (C.c_int, [_p]),
     # include/tnx_random.h
     "tnx_threefry2x32": (C.c_int, [C.c_uint32] * 4 + [C.POINTER(C.c_uint32)]),
     "tnx_random_normal_blocks": (C.c_int, [_i32, C.POINTER(C.c_uint32), _PI64, _PI64, _i32, C.c_double, _p, _p]),

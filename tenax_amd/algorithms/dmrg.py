@@ -317,12 +317,70 @@ class TwoSiteProblem:
                                    compute, _lib.C.byref(e), _lib.C.byref(n), _lib.stream_handle()))
         return e.value, n.value
 
+    def lanczos_graph(self, max_iter: int, tol: float, reorth: bool, compute: int) -> "LanczosGraph":
+        """The device Lanczos solve of :meth:`lanczos` recorded once as a hipGraph
+        (tnx_lanczos_graph_create), for repeated solves on this structure."""
+        return LanczosGraph(self, max_iter, tol, reorth, compute)
+
     def extract(self, present: np.ndarray) -> SymmetricTensor:
         sel = np.nonzero(present)[0]
         keys, shapes = self.keys[sel], self.shapes[sel]
         offs, _ = SymmetricTensor._layout(shapes)
         gi = torch.from_numpy(_ranges(self.offsets, self.shapes, sel)).to(self.x.device)
         return SymmetricTensor._wrap(self.indices, keys, shapes, offs, self.x[gi])
+
+
+class LanczosGraph:
+    """A captured device Lanczos solve on a :class:`TwoSiteProblem` (no reference
+    counterpart: dmrg.py:978-1045 is a Python loop).  ``launch()`` replays every
+    kernel of the solve with one graph launch on the current stream, with the
+    problem's vector ``x`` as start vector and output; ``result()`` waits and
+    returns (energy, n_steps).  ``update(x=...)`` re-records it for another start
+    vector buffer or after the plan's tiling changed.  Bitwise the same as
+    :meth:`TwoSiteProblem.lanczos`."""
+
+    def __init__(self, prob: TwoSiteProblem, max_iter: int, tol: float, reorth: bool, compute: int):
+        self.prob = prob  # keeps the plan and the operand buffers alive
+        self.x = prob.x
+        L, Wl, Wr, R = prob.envs
+        lib = _lib.load()
+        ptr = _lib.C.c_void_p()
+        _lib.check(lib.tnx_lanczos_graph_create(prob.plan.ptr, _lib.dptr(L.data), _lib.dptr(Wl.data),
+                                                _lib.dptr(Wr.data), _lib.dptr(R.data), _lib.dptr(self.x),
+                                                int(max_iter), float(tol), int(bool(reorth)), int(compute),
+                                                _lib.stream_handle(), _lib.C.byref(ptr)))
+        self.ptr = ptr
+
+    def update(self, x: torch.Tensor | None = None):
+        if x is not None:
+            if x.shape != self.prob.x.shape or x.dtype != torch.float64 or not x.is_contiguous():
+                raise ValueError("start vector must be a contiguous float64 tensor of the problem's size")
+            self.x = x
+        L, Wl, Wr, R = self.prob.envs
+        _lib.check(_lib.load().tnx_lanczos_graph_update(self.ptr, _lib.dptr(L.data), _lib.dptr(Wl.data),
+                                                        _lib.dptr(Wr.data), _lib.dptr(R.data), _lib.dptr(self.x),
+                                                        _lib.stream_handle()))
+
+    def launch(self):
+        _lib.check(_lib.load().tnx_graph_launch(self.ptr, _lib.stream_handle()))
+
+    def result(self):
+        e = _lib.C.c_double()
+        n = _lib.C.c_int32()
+        _lib.check(_lib.load().tnx_lanczos_graph_result(self.ptr, _lib.stream_handle(), _lib.C.byref(e),
+                                                        _lib.C.byref(n)))
+        return e.value, n.value
+
+    def close(self):
+        if getattr(self, "ptr", None) is not None and self.ptr.value:
+            _lib.load(require_gpu=False).tnx_graph_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _theta_padded(site_l, site_r):

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <mutex>
+#include <memory>
 #include <unordered_map>
 
 #include "../../include/tnx_dmrg.h"
@@ -1214,11 +1215,6 @@ __global__ void lz_combine_kernel(double *__restrict__ x, const double *__restri
   }
 }
 
-__global__ void lz_copy_kernel(double *__restrict__ x, const double *__restrict__ v, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    x[i] = v[i];
-}
-
 // Sturm count: number of eigenvalues of T strictly below x.
 __device__ int sturm_count(const double *a, const double *b, int m, double x, double pivmin) {
   int cnt = 0;
@@ -1324,29 +1320,46 @@ struct LanczosWorkspace {
 static std::mutex g_lz_mutex;
 static LanczosWorkspace g_lz;
 
-static void lanczos_run(const FusedPlan &P, const double *L, const double *Wl, const double *Wr, const double *R,
-                        double *theta, int max_iter, double tol, int reorth, int compute, double *energy,
-                        int *nsteps, hipStream_t s) {
+__global__ void lz_init_state_kernel(LzState *__restrict__ st) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *st = LzState{0, 0, 0.0, 0.0};
+}
+
+// dmrg.py:1029-1030: a single step returns the normalised start vector itself.
+__global__ void lz_single_step_kernel(double *__restrict__ x, const double *__restrict__ v, int64_t n,
+                                      const LzState *__restrict__ st) {
+  if (st->nsteps != 1) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = v[i];
+}
+
+static void lanczos_reserve(LanczosWorkspace &W, int64_t n, int max_iter) {
   TNX_REQUIRE(max_iter >= 1, TNX_ERR_ARG, "max_iter must be >= 1");
-  const int64_t n = P.in_total;
-  std::lock_guard<std::mutex> lk(g_lz_mutex);
-  LanczosWorkspace &W = g_lz;
   const int64_t need_v = (int64_t)(max_iter + 1) * std::max<int64_t>(n, 1);
   if ((int64_t)W.V.n < need_v) W.V.alloc(need_v);
   const int64_t nscal = 3 * (max_iter + 2) + (max_iter + 2) + 5 * (max_iter + 2) + (int64_t)LZ_G * (max_iter + 2);
   if ((int64_t)W.scal.n < nscal) W.scal.alloc(nscal);
   if (!W.st.p) W.st.alloc(1);
+  W.n = n;
+  W.max_iter = max_iter;
+}
+
+// The whole solve as device work on stream s (no host synchronisation, no
+// host-memory copies, no allocations: it can be captured into a hipGraph).
+// Leaves the eigenvector in theta and energy / step count in W.st.
+static void lanczos_enqueue(const FusedPlan &P, const double *L, const double *Wl, const double *Wr,
+                            const double *R, double *theta, int max_iter, double tol, int reorth, int compute,
+                            LanczosWorkspace &W, hipStream_t s) {
+  const int64_t n = P.in_total;
   double *alpha = W.scal.p;
   double *beta = alpha + (max_iter + 2);
   double *cvec = beta + (max_iter + 2);
   double *rc = cvec + (max_iter + 2);
   double *work = rc + (max_iter + 2);
   double *part = work + 5 * (max_iter + 2);
-  static const LzState init{0, 0, 0.0, 0.0};
-  TNX_CHECK_HIP(hipMemcpyAsync(W.st.p, &init, sizeof(LzState), hipMemcpyHostToDevice, s));
   double *V = W.V.p;
   const dim3 vg(std::min<int64_t>(std::max<int64_t>((n + 255) / 256, 1), 1024)), vb(256);
   const dim3 rg(LZ_G);
+  hipLaunchKernelGGL(lz_init_state_kernel, dim3(1), dim3(64), 0, s, W.st.p);
   // v0 = theta / (|theta| + 1e-15)
   hipLaunchKernelGGL(dot_partial_kernel, rg, vb, 0, s, theta, theta, n, part, (const LzState *)nullptr);
   hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), vb, 0, s, part, LZ_G, 0, 0, tol, alpha, beta, W.st.p);
@@ -1375,17 +1388,25 @@ static void lanczos_run(const FusedPlan &P, const double *L, const double *Wl, c
   hipLaunchKernelGGL(dot_partial_kernel, rg, vb, 0, s, theta, theta, n, part, (const LzState *)nullptr);
   hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), vb, 0, s, part, LZ_G, 3, 0, tol, alpha, beta, W.st.p);
   hipLaunchKernelGGL(lz_scale_kernel, vg, vb, 0, s, theta, theta, n, W.st.p, 0);
+  hipLaunchKernelGGL(lz_single_step_kernel, vg, vb, 0, s, theta, V, n, W.st.p);
   TNX_CHECK_HIP(hipGetLastError());
+}
+
+static void lanczos_finish(const LanczosWorkspace &W, double *energy, int *nsteps, hipStream_t s) {
   LzState out{};
   TNX_CHECK_HIP(hipMemcpyAsync(&out, W.st.p, sizeof(LzState), hipMemcpyDeviceToHost, s));
   TNX_CHECK_HIP(hipStreamSynchronize(s));
-  if (out.nsteps == 1) {
-    // dmrg.py:1029-1030: a single step returns the normalised start vector
-    hipLaunchKernelGGL(lz_copy_kernel, vg, vb, 0, s, theta, V, n);
-    TNX_CHECK_HIP(hipStreamSynchronize(s));
-  }
   *energy = out.energy;
   *nsteps = out.nsteps;
+}
+
+static void lanczos_run(const FusedPlan &P, const double *L, const double *Wl, const double *Wr, const double *R,
+                        double *theta, int max_iter, double tol, int reorth, int compute, double *energy,
+                        int *nsteps, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_lz_mutex);
+  lanczos_reserve(g_lz, P.in_total, max_iter);
+  lanczos_enqueue(P, L, Wl, Wr, R, theta, max_iter, tol, reorth, compute, g_lz, s);
+  lanczos_finish(g_lz, energy, nsteps, s);
 }
 
 GemmProgram *fused_program(tnx_plan_s *plan, int stage) {
@@ -1541,4 +1562,131 @@ int tnx_lanczos(const tnx_plan *matvec_plan, const double *L, const double *Wl, 
   });
 }
 
+// ------------------------------------------------------------------ captured Lanczos solve
 }  // extern "C"
+
+struct tnx_graph_s {
+  const tnx::FusedPlan *plan = nullptr;
+  uint64_t gen = 0;  // plan generation the captured work lists belong to
+  int max_iter = 0, reorth = 0, compute = 0;
+  double tol = 0.0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  tnx::LanczosWorkspace ws;  // the graph's own Krylov basis and scalars
+  ~tnx_graph_s() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+namespace tnx {
+
+// Record the solve on a private non-blocking stream (default priority); the
+// recorded work runs only when the instantiated graph is launched.
+static hipGraph_t capture_lanczos(tnx_graph_s &G, const double *L, const double *Wl, const double *Wr,
+                                  const double *R, double *theta) {
+  hipStream_t cs = nullptr;
+  TNX_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(cs);
+    TNX_CHECK_HIP(e);
+  }
+  hipGraph_t g = nullptr;
+  try {
+    lanczos_enqueue(*G.plan, L, Wl, Wr, R, theta, G.max_iter, G.tol, G.reorth, G.compute, G.ws, cs);
+  } catch (...) {
+    (void)hipStreamEndCapture(cs, &g);
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipStreamDestroy(cs);
+    throw;
+  }
+  e = hipStreamEndCapture(cs, &g);
+  (void)hipStreamDestroy(cs);
+  TNX_CHECK_HIP(e);
+  return g;
+}
+
+}  // namespace tnx
+
+extern "C" {
+
+int tnx_lanczos_graph_create(const tnx_plan *matvec_plan, const double *L, const double *Wl, const double *Wr,
+                             const double *R, double *theta, int32_t max_iter, double tol, int32_t reorth,
+                             int32_t compute, void *stream, tnx_graph **graph) {
+  return guarded([&] {
+    TNX_REQUIRE(matvec_plan && matvec_plan->kind == PlanKind::Matvec, TNX_ERR_ARG, "not a matvec plan");
+    TNX_REQUIRE(graph, TNX_ERR_ARG, "null output");
+    TNX_REQUIRE(compute >= 0 && compute <= 2, TNX_ERR_DTYPE, "unknown compute mode");
+    const FusedPlan &P = static_cast<const FusedPlan &>(*matvec_plan);
+    auto G = std::make_unique<tnx_graph_s>();
+    G->plan = &P;
+    G->gen = P.generation;
+    G->max_iter = max_iter;
+    G->tol = tol;
+    G->reorth = reorth ? 1 : 0;
+    G->compute = compute;
+    lanczos_reserve(G->ws, P.in_total, max_iter);
+    TNX_CHECK_HIP(hipStreamSynchronize((hipStream_t)stream));  // workspace ready before any replay
+    G->graph = capture_lanczos(*G, L, Wl, Wr, R, theta);
+    TNX_CHECK_HIP(hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0));
+    *graph = G.release();
+  });
+}
+
+int tnx_lanczos_graph_update(tnx_graph *graph, const double *L, const double *Wl, const double *Wr,
+                             const double *R, double *theta, void *stream) {
+  return guarded([&] {
+    TNX_REQUIRE(graph, TNX_ERR_ARG, "null graph");
+    tnx_graph_s &G = *graph;
+    TNX_CHECK_HIP(hipStreamSynchronize((hipStream_t)stream));  // no replay of the old graph in flight
+    hipGraph_t ng = capture_lanczos(G, L, Wl, Wr, R, theta);
+    hipGraphNode_t err_node = nullptr;
+    hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+    const hipError_t e = hipGraphExecUpdate(G.exec, ng, &err_node, &res);
+    if (e != hipSuccess || res != hipGraphExecUpdateSuccess) {
+      (void)hipGetLastError();
+      (void)hipGraphExecDestroy(G.exec);
+      G.exec = nullptr;
+      const hipError_t ei = hipGraphInstantiate(&G.exec, ng, nullptr, nullptr, 0);
+      if (ei != hipSuccess) {
+        (void)hipGraphDestroy(ng);
+        TNX_CHECK_HIP(ei);
+      }
+    }
+    (void)hipGraphDestroy(G.graph);
+    G.graph = ng;
+    G.gen = G.plan->generation;
+  });
+}
+
+int tnx_graph_launch(tnx_graph *graph, void *stream) {
+  return guarded([&] {
+    TNX_REQUIRE(graph && graph->exec, TNX_ERR_ARG, "null graph");
+    TNX_REQUIRE(graph->plan->generation == graph->gen, TNX_ERR_ARG,
+                "the plan's tiling changed after capture; call tnx_lanczos_graph_update first");
+    TNX_CHECK_HIP(hipGraphLaunch(graph->exec, (hipStream_t)stream));
+  });
+}
+
+int tnx_lanczos_graph_result(tnx_graph *graph, void *stream, double *energy, int32_t *n_steps) {
+  return guarded([&] {
+    TNX_REQUIRE(graph, TNX_ERR_ARG, "null graph");
+    TNX_REQUIRE(energy && n_steps, TNX_ERR_ARG, "null output");
+    int ns = 0;
+    lanczos_finish(graph->ws, energy, &ns, (hipStream_t)stream);
+    *n_steps = ns;
+  });
+}
+
+int tnx_graph_free(tnx_graph *graph) {
+  return guarded([&] {
+    if (graph) {
+      (void)hipDeviceSynchronize();
+      delete graph;
+    }
+  });
+}
+
+}  // extern "C"
+

@@ -221,6 +221,9 @@ struct tnx_plan_base {
   std::vector<int64_t> out_offsets;
   double flops = 0.0, bytes = 0.0;
   int32_t n_pairs = 0, n_tasks = 0;
+  // bumped whenever the device work lists are rebuilt (tnx_plan_set_tiling);
+  // captured graphs refuse to replay work lists of an older generation
+  uint64_t generation = 0;
 };
 
 }  // namespace tnx

@@ -1012,6 +1012,7 @@ int tnx_plan_set_tiling(tnx_plan *plan, int32_t stage, int32_t shape, int32_t kc
     c.kchunk = kchunk;
     c.xcd = xcd ? 1 : 0;
     set_tiling(*g, c, (hipStream_t)stream);
+    ++plan->generation;
   });
 }
 

@@ -166,6 +166,51 @@ def test_device_lanczos_matches_oracle(gpu, state8, reorth):
     assert abs(abs(dot) - 1.0) < 1e-8
 
 
+@pytest.mark.parametrize("reorth", [False, True])
+def test_lanczos_graph_replays_match_eager(gpu, state8, reorth):
+    """The captured solve (tnx_lanczos_graph_*) reproduces the eager device
+    solve bit for bit on every replay, with eager work and synchronisations
+    between replays; it follows a new start-vector buffer after an update, and
+    refuses to replay after the plan's tiling changed until it is updated."""
+    mpo, mps = state8
+    prob, _ = _problem(mpo, mps, 3)
+    x0 = prob.x.clone()
+    e_ref, n_ref = prob.lanczos(30, 1e-12, reorth, 0)
+    v_ref = prob.x.clone()
+    prob.x.copy_(x0)
+    g = prob.lanczos_graph(30, 1e-12, reorth, 0)
+    try:
+        for _ in range(3):
+            prob.x.copy_(x0)
+            prob.matvec(x0, 0)  # eager kernels on the same plan between replays
+            torch.cuda.synchronize()
+            g.launch()
+            e, n = g.result()
+            assert (e, n) == (e_ref, n_ref)
+            assert torch.equal(prob.x, v_ref)
+        x2 = x0.clone()
+        g.update(x=x2)
+        g.launch()
+        e, n = g.result()
+        assert (e, n) == (e_ref, n_ref)
+        assert torch.equal(x2, v_ref)
+        prob.plan.set_tiling(3, 0, 1, 1)
+        with pytest.raises(Exception, match="tiling changed"):
+            g.launch()
+        prob.x.copy_(x0)
+        e_t, n_t = prob.lanczos(30, 1e-12, reorth, 0)
+        v_t = prob.x.clone()
+        x2.copy_(x0)
+        g.update()
+        g.launch()
+        e, n = g.result()
+        assert (e, n) == (e_t, n_t)
+        assert torch.equal(x2, v_t)
+    finally:
+        g.close()
+        prob.plan.set_tiling(3, 0, 0, 0)
+
+
 @pytest.mark.parametrize("L,tgt", [(4, 0), (6, 0), (4, 2)])
 def test_dmrg_matches_ed(gpu, L, tgt):
     from tenax_amd import DMRGConfig, build_mpo_heisenberg, build_random_symmetric_mps, dmrg
