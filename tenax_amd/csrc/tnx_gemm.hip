@@ -55,54 +55,78 @@ __device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off)
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int32_t)off, 0, 0));
 }
 
-// A operand: MK = a[m*lda+k] (4 threads per row of 16 k, 4 consecutive k
-// each), KM = a[k*lda+m] (one element per thread per k row).
+// Per-thread byte offsets of a thread's operand elements, relative to the
+// first row (KM, KN) or column (MK) of the K tile.  They depend only on the
+// block pair, so they are computed when the pair changes and stay in
+// registers across its K tiles; each tile only moves the buffer descriptor's
+// base (scalar arithmetic).
+//   A MK: 4 threads per row of 16 k, 4 consecutive k each;
+//   A KM and B: one element per thread per k row, consecutive lanes on
+//   consecutive m / n (coalesced 512-byte rows, conflict-free LDS stores),
+//   rows RPP apart.
 template <int ALAY, int BM>
-__device__ __forceinline__ void load_a(const double *__restrict__ A, const GemmPair &p, int M, int m0, int k0,
-                                       int tid, double (&r)[BM / 64][4]) {
-  const int32_t nb = (int32_t)(8 * (ALAY == LAY_MK ? (int64_t)(M - 1) * p.lda + p.K : (int64_t)(p.K - 1) * p.lda + M));
-  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void *)(A + p.a_off), (short)0, nb, BUF_DWORD3);
+__device__ __forceinline__ void a_offsets(const GemmPair &p, int m0, int tid, uint32_t (&o)[BM / 64][4]) {
 #pragma unroll
-  for (int q = 0; q < BM / 64; ++q) {
-    if (ALAY == LAY_MK) {
-      // rows past M lie past the block's end; k >= K selects the end offset
-      const uint32_t m = (uint32_t)(m0 + (tid >> 2) + 64 * q);
-      const int kq = k0 + ((tid & 3) << 2);
-      const uint32_t run = 8u * (m * (uint32_t)p.lda + (uint32_t)kq);
-      const int lim = p.K - kq;
+  for (int q = 0; q < BM / 64; ++q)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) r[q][i] = buf_ld(rs, i < lim ? run + 8u * i : (uint32_t)nb);
-    } else {
-      // one element per thread per k row, consecutive lanes on consecutive m
-      // (coalesced 512-byte rows, conflict-free LDS stores); rows k >= K lie
-      // past the block's end
-      constexpr int RPP = 256 / BM;  // k rows per pass
-      const uint32_t m = (uint32_t)(m0 + tid % BM);
-      const uint32_t kb = (uint32_t)(k0 + tid / BM);
-      const uint32_t base = 8u * (kb * (uint32_t)p.lda + m);
-      const uint32_t step = 8u * RPP * (uint32_t)p.lda;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) r[q][i] = buf_ld(rs, base + step * (uint32_t)(4 * q + i));
+    for (int i = 0; i < 4; ++i) {
+      if (ALAY == LAY_MK) {
+        const uint32_t m = (uint32_t)(m0 + (tid >> 2) + 64 * q);
+        o[q][i] = 8u * (m * (uint32_t)p.lda + (uint32_t)(((tid & 3) << 2) + i));
+      } else {
+        constexpr int RPP = 256 / BM;
+        o[q][i] = 8u * ((uint32_t)(tid / BM + RPP * (4 * q + i)) * (uint32_t)p.lda + (uint32_t)(m0 + tid % BM));
+      }
     }
-  }
 }
-
-// B operand is always K x N row-major (KN): b[k*ldb + n].
 template <int BN>
-__device__ __forceinline__ void load_b(const double *__restrict__ B, const GemmPair &p, int N, int n0, int k0,
-                                       int tid, double (&r)[BN / 64][4]) {
-  constexpr int RPP = 256 / BN;  // k rows per pass; one element per thread per row
-  const int32_t nb = (int32_t)(8 * ((int64_t)(p.K - 1) * p.ldb + N));
-  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void *)(B + p.b_off), (short)0, nb, BUF_DWORD3);
-  // rows k >= K lie past the block's end; columns past N are never stored
-  const uint32_t n = (uint32_t)(n0 + tid % BN);
-  const uint32_t kb = (uint32_t)(k0 + tid / BN);
-  const uint32_t base = 8u * (kb * (uint32_t)p.ldb + n);
-  const uint32_t step = 8u * RPP * (uint32_t)p.ldb;
+__device__ __forceinline__ void b_offsets(const GemmPair &p, int n0, int tid, uint32_t (&o)[BN / 64][4]) {
+  constexpr int RPP = 256 / BN;
 #pragma unroll
   for (int q = 0; q < BN / 64; ++q)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) r[q][i] = buf_ld(rs, base + step * (uint32_t)(4 * q + i));
+    for (int i = 0; i < 4; ++i)
+      o[q][i] = 8u * ((uint32_t)(tid / BN + RPP * (4 * q + i)) * (uint32_t)p.ldb + (uint32_t)(n0 + tid % BN));
+}
+
+// A tile at K offset k0 of pair p (M rows of the output block): the
+// descriptor starts at the tile's first column (MK) or row (KM) and ends at
+// the block's end.  MK: rows past M lie past the end; k >= K within a row
+// selects an out-of-range offset.  KM: rows k >= K lie past the end.
+template <int ALAY, int BM>
+__device__ __forceinline__ void load_a(const double *__restrict__ A, const GemmPair &p, int M, int k0, int tid,
+                                       const uint32_t (&o)[BM / 64][4], double (&r)[BM / 64][4]) {
+  const int32_t shift = ALAY == LAY_MK ? k0 : k0 * p.lda;  // elements; < 2^28 (finalize_program)
+  const int32_t nb = (int32_t)(8 * (ALAY == LAY_MK ? (int64_t)(M - 1) * p.lda + p.K - k0
+                                                    : (int64_t)(p.K - 1 - k0) * p.lda + M));
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc((void *)(A + p.a_off + shift), (short)0, nb, BUF_DWORD3);
+  if (ALAY == LAY_MK) {
+    const int lim = p.K - k0 - ((tid & 3) << 2);
+#pragma unroll
+    for (int q = 0; q < BM / 64; ++q)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) r[q][i] = buf_ld(rs, i < lim ? o[q][i] : (uint32_t)nb);
+  } else {
+#pragma unroll
+    for (int q = 0; q < BM / 64; ++q)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) r[q][i] = buf_ld(rs, o[q][i]);
+  }
+}
+
+// B operand is always K x N row-major (KN): b[k*ldb + n]; rows k >= K lie past
+// the block's end, columns past N are never stored.
+template <int BN>
+__device__ __forceinline__ void load_b(const double *__restrict__ B, const GemmPair &p, int N, int k0,
+                                       const uint32_t (&o)[BN / 64][4], double (&r)[BN / 64][4]) {
+  const int32_t nb = (int32_t)(8 * ((int64_t)(p.K - 1 - k0) * p.ldb + N));
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc((void *)(B + p.b_off + (int64_t)k0 * p.ldb), (short)0, nb, BUF_DWORD3);
+#pragma unroll
+  for (int q = 0; q < BN / 64; ++q)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r[q][i] = buf_ld(rs, o[q][i]);
 }
 
 // LDS positions ([k][m] and [k][n], pitch BM+1 / BN+1) of a thread's staged values.
@@ -344,17 +368,24 @@ __global__ __launch_bounds__(GEMM_THREADS) __attribute__((amdgpu_waves_per_eu(
 
   // Load cursor over the flattened (pair, k-tile) sequence.
   int lk0 = skip * GEMM_BK;
+  uint32_t aoff[BM / 64][4], boff[BN / 64][4];
+  a_offsets<ALAY, BM>(p, t.m0, tid, aoff);
+  b_offsets<BN>(p, t.n0, tid, boff);
   auto advance = [&]() {
     lk0 += GEMM_BK;
     if (lk0 >= p.K) {
       lk0 = 0;
-      if (pi + 1 < o.pair_end) p = pairs[pi + 1];
+      if (pi + 1 < o.pair_end) {
+        p = pairs[pi + 1];
+        a_offsets<ALAY, BM>(p, t.m0, tid, aoff);
+        b_offsets<BN>(p, t.n0, tid, boff);
+      }
       ++pi;
     }
   };
   auto load = [&](Stage<BM, BN> &st) {
-    load_a<ALAY, BM>(A, p, o.M, t.m0, lk0, tid, st.a);
-    load_b<BN>(B, p, o.N, t.n0, lk0, tid, st.b);
+    load_a<ALAY, BM>(A, p, o.M, lk0, tid, aoff, st.a);
+    load_b<BN>(B, p, o.N, lk0, boff, st.b);
     st.neg_hi = p.neg ? 0x80000000u : 0u;
     advance();
   };
