@@ -54,13 +54,18 @@ constexpr int32_t BUF_DWORD3 = 0x00020000;  // raw dword access, gfx9 family
 __device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int32_t)off, 0, 0));
 }
+// Two consecutive doubles (one 128-bit load).
+__device__ __forceinline__ double2 buf_ld2(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, (int32_t)off, 0, 0));
+}
 
 // Per-thread byte offsets of a thread's operand elements, relative to the
 // first row (KM, KN) or column (MK) of the K tile.  They depend only on the
 // block pair, so they are computed when the pair changes and stay in
 // registers across its K tiles; each tile only moves the buffer descriptor's
 // base (scalar arithmetic).
-//   A MK: 4 threads per row of 16 k, 4 consecutive k each;
+//   A MK: 8 threads per row of 16 k, 2 consecutive k each (one 128-bit
+//   load per row), rows 32 apart;
 //   A KM and B: one element per thread per k row, consecutive lanes on
 //   consecutive m / n (coalesced 512-byte rows, conflict-free LDS stores),
 //   rows RPP apart.
@@ -71,8 +76,10 @@ __device__ __forceinline__ void a_offsets(const GemmPair &p, int m0, int tid, ui
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       if (ALAY == LAY_MK) {
-        const uint32_t m = (uint32_t)(m0 + (tid >> 2) + 64 * q);
-        o[q][i] = 8u * (m * (uint32_t)p.lda + (uint32_t)(((tid & 3) << 2) + i));
+        // element e = 4q + i is half h = e & 1 of 128-bit load j = e >> 1
+        const int e = 4 * q + i;
+        const uint32_t m = (uint32_t)(m0 + (tid >> 3) + 32 * (e >> 1));
+        o[q][i] = 8u * (m * (uint32_t)p.lda + (uint32_t)(((tid & 7) << 1) + (e & 1)));
       } else {
         constexpr int RPP = 256 / BM;
         o[q][i] = 8u * ((uint32_t)(tid / BM + RPP * (4 * q + i)) * (uint32_t)p.lda + (uint32_t)(m0 + tid % BM));
@@ -102,11 +109,24 @@ __device__ __forceinline__ void load_a(const double *__restrict__ A, const GemmP
   const __amdgpu_buffer_rsrc_t rs =
       __builtin_amdgcn_make_buffer_rsrc((void *)(A + p.a_off + shift), (short)0, nb, BUF_DWORD3);
   if (ALAY == LAY_MK) {
-    const int lim = p.K - k0 - ((tid & 3) << 2);
+    if (k0 + GEMM_BK <= p.K) {  // wave-uniform: the whole K tile lies inside the pair
 #pragma unroll
-    for (int q = 0; q < BM / 64; ++q)
+      for (int e = 0; e < BM / 32; ++e) {
+        const double2 v = buf_ld2(rs, o[e >> 1][(e & 1) * 2]);
+        r[(2 * e) >> 2][(2 * e) & 3] = v.x;
+        r[(2 * e + 1) >> 2][(2 * e + 1) & 3] = v.y;
+      }
+    } else {
+      // the pair's last K tile: element loads, k >= K selects an out-of-range
+      // offset (a 128-bit load could straddle the block's end)
+      const int lim = p.K - k0 - ((tid & 7) << 1);  // valid k of this thread's pair: <= 0, 1 or >= 2
 #pragma unroll
-      for (int i = 0; i < 4; ++i) r[q][i] = buf_ld(rs, i < lim ? o[q][i] : (uint32_t)nb);
+      for (int e = 0; e < BM / 32; ++e) {
+        const uint32_t base = o[e >> 1][(e & 1) * 2];
+        r[(2 * e) >> 2][(2 * e) & 3] = buf_ld(rs, lim > 0 ? base : (uint32_t)nb);
+        r[(2 * e + 1) >> 2][(2 * e + 1) & 3] = buf_ld(rs, lim > 1 ? base + 8u : (uint32_t)nb);
+      }
+    }
   } else {
 #pragma unroll
     for (int q = 0; q < BM / 64; ++q)
@@ -133,7 +153,10 @@ __device__ __forceinline__ void load_b(const double *__restrict__ B, const GemmP
 template <int ALAY, int BM>
 __device__ __forceinline__ int a_lds(int tid, int q, int i) {
   constexpr int AP = BM + 1;
-  if (ALAY == LAY_MK) return (((tid & 3) << 2) + i) * AP + (tid >> 2) + 64 * q;
+  if (ALAY == LAY_MK) {
+    const int e = 4 * q + i;
+    return (((tid & 7) << 1) + (e & 1)) * AP + (tid >> 3) + 32 * (e >> 1);
+  }
   constexpr int RPP = 256 / BM;
   return (tid / BM + RPP * (4 * q + i)) * AP + tid % BM;
 }
