@@ -1199,6 +1199,7 @@ __global__ void lz_reorth_kernel(double *__restrict__ w, const double *__restric
   if (st->stopped) return;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double t = w[i];
+#pragma unroll 8
     for (int j = 0; j < nv; ++j) t = t - V[(int64_t)j * n + i] * c[j];
     w[i] = t;
   }
@@ -1210,6 +1211,8 @@ __global__ void lz_combine_kernel(double *__restrict__ x, const double *__restri
   const int m = st->nsteps;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double t = V[i] * c[0];
+    // unrolled so several basis loads are in flight; the sum keeps its order
+#pragma unroll 8
     for (int k = 1; k < m; ++k) t = t + V[(int64_t)k * n + i] * c[k];
     x[i] = t;
   }
@@ -1230,21 +1233,48 @@ __device__ int sturm_count(const double *a, const double *b, int m, double x, do
 }
 
 // Lowest eigenpair of the m x m symmetric tridiagonal T (diag alpha[0..m),
-// off-diagonal beta[1..m)) by bisection and inverse iteration.  One thread.
+// off-diagonal beta[1..m)) by Sturm multisection and inverse iteration, one
+// wave.  For m <= LZ_TRI_LDS every array lives in LDS, so the serial Sturm
+// and elimination recurrences do not wait on global memory (otherwise they
+// run on alpha / beta / work in global memory).  Each multisection round
+// evaluates the count at 64 interior points of the bracket and keeps the
+// sub-interval holding the lowest eigenvalue (a 65x shrink per round).
 // Writes c[0..m) and st->energy; m = st->nsteps (set to max_iter when no stop).
-__global__ void lz_tridiag_kernel(const double *__restrict__ alpha, const double *__restrict__ beta,
-                                  double *__restrict__ c, double *__restrict__ work, LzState *__restrict__ st,
-                                  int max_iter) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (!st->stopped) st->nsteps = max_iter;
-  const int m = st->nsteps;
+constexpr int LZ_TRI_LDS = 512;
+__global__ void __launch_bounds__(64) lz_tridiag_kernel(const double *__restrict__ alpha,
+                                                        const double *__restrict__ beta, double *__restrict__ c,
+                                                        double *__restrict__ work, LzState *__restrict__ st,
+                                                        int max_iter) {
+  __shared__ double lds[8 * LZ_TRI_LDS];
+  __shared__ int sm;
+  const int lane = threadIdx.x;
+  if (lane == 0) {
+    if (!st->stopped) st->nsteps = max_iter;
+    sm = st->nsteps;
+  }
+  __syncthreads();
+  const int m = sm;
   if (m == 1) {
-    st->energy = alpha[0];
-    c[0] = 1.0;
+    if (lane == 0) {
+      st->energy = alpha[0];
+      c[0] = 1.0;
+    }
     return;
   }
-  const double *a = alpha;
-  const double *b = beta;  // b[i] couples i-1 and i, i = 1..m-1
+  const bool staged = m <= LZ_TRI_LDS;
+  const double *a = alpha, *b = beta;  // b[i] couples i-1 and i, i = 1..m-1
+  double *wk = work, *cv = c;
+  if (staged) {
+    for (int i = lane; i < m; i += 64) {
+      lds[i] = alpha[i];
+      lds[LZ_TRI_LDS + i] = beta[i];
+    }
+    a = lds;
+    b = lds + LZ_TRI_LDS;
+    wk = lds + 2 * LZ_TRI_LDS;  // five work arrays of LZ_TRI_LDS
+    cv = lds + 7 * LZ_TRI_LDS;
+  }
+  __syncthreads();
   double lo = 1e300, hi = -1e300, tnorm = 0.0;
   for (int i = 0; i < m; ++i) {
     const double r = (i > 0 ? fabs(b[i]) : 0.0) + (i + 1 < m ? fabs(b[i + 1]) : 0.0);
@@ -1256,17 +1286,32 @@ __global__ void lz_tridiag_kernel(const double *__restrict__ alpha, const double
   const double pivmin = fmax(tnorm * eps * eps, 1e-300);
   lo -= 2 * eps * tnorm + pivmin;
   hi += 2 * eps * tnorm + pivmin;
-  for (int it = 0; it < 2000; ++it) {
-    const double mid = 0.5 * (lo + hi);
-    if (mid <= lo || mid >= hi) break;
-    if (sturm_count(a, b, m, mid, pivmin) >= 1) hi = mid; else lo = mid;
+  for (int it = 0; it < 200; ++it) {
     if (hi - lo <= 2 * eps * fmax(fabs(lo), fabs(hi))) break;
+    const double x = lo + (hi - lo) * (double)(lane + 1) / 65.0;
+    const bool below = x > lo && x < hi && sturm_count(a, b, m, x, pivmin) >= 1;
+    // the lowest point with an eigenvalue below it is the new upper end, the
+    // point before it (or lo) the new lower end; wave-wide through a ballot
+    const unsigned long long mask = __ballot(below);
+    double nlo = lo, nhi = hi;
+    if (mask) {
+      const int f = __ffsll((long long)mask) - 1;
+      nhi = lo + (hi - lo) * (double)(f + 1) / 65.0;
+      if (f > 0) nlo = lo + (hi - lo) * (double)f / 65.0;
+    } else {
+      nlo = lo + (hi - lo) * 64.0 / 65.0;
+    }
+    if (nlo <= lo && nhi >= hi) break;  // no representable interior point left
+    lo = nlo;
+    hi = nhi;
   }
+  if (lane != 0) return;
   const double lam = 0.5 * (lo + hi);
   st->energy = lam;
   // inverse iteration: (T - lam) y = x, pivoted tridiagonal elimination
-  double *dl = work, *d = work + m, *du = work + 2 * m, *du2 = work + 3 * m, *y = work + 4 * m;
-  for (int i = 0; i < m; ++i) c[i] = 1.0 / sqrt((double)m);
+  const int ld = staged ? LZ_TRI_LDS : m;
+  double *dl = wk, *d = wk + ld, *du = wk + 2 * ld, *du2 = wk + 3 * ld, *y = wk + 4 * ld;
+  for (int i = 0; i < m; ++i) cv[i] = 1.0 / sqrt((double)m);
   const double tiny = fmax(eps * tnorm, 1e-300);
   for (int iter = 0; iter < 4; ++iter) {
     for (int i = 0; i < m; ++i) {
@@ -1274,7 +1319,7 @@ __global__ void lz_tridiag_kernel(const double *__restrict__ alpha, const double
       du[i] = (i + 1 < m) ? b[i + 1] : 0.0;
       dl[i] = (i + 1 < m) ? b[i + 1] : 0.0;
       du2[i] = 0.0;
-      y[i] = c[i];
+      y[i] = cv[i];
     }
     for (int i = 0; i < m - 1; ++i) {
       if (fabs(d[i]) >= fabs(dl[i])) {
@@ -1286,13 +1331,13 @@ __global__ void lz_tridiag_kernel(const double *__restrict__ alpha, const double
       } else {
         const double f = d[i] / dl[i];
         d[i] = dl[i];
-        const double t = d[i + 1];
-        d[i + 1] = du[i] - f * t;
+        const double tmp = d[i + 1];
+        d[i + 1] = du[i] - f * tmp;
         if (i < m - 2) {
           du2[i] = du[i + 1];
           du[i + 1] = -f * du2[i];
         }
-        du[i] = t;
+        du[i] = tmp;
         const double ty = y[i];
         y[i] = y[i + 1];
         y[i + 1] = ty - f * y[i + 1];
@@ -1305,8 +1350,10 @@ __global__ void lz_tridiag_kernel(const double *__restrict__ alpha, const double
     double nrm = 0.0;
     for (int i = 0; i < m; ++i) nrm += y[i] * y[i];
     nrm = sqrt(nrm);
-    for (int i = 0; i < m; ++i) c[i] = y[i] / nrm;
+    for (int i = 0; i < m; ++i) cv[i] = y[i] / nrm;
   }
+  if (staged)
+    for (int i = 0; i < m; ++i) c[i] = cv[i];
 }
 
 struct LanczosWorkspace {
