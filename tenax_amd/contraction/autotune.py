@@ -49,6 +49,12 @@ def _time(run, reps: int) -> float:
     return s.elapsed_time(e) / reps
 
 
+def cached(stage: int, flops: float, kind: str):
+    """The tiling already chosen for this (kind, stage, work bucket), or None."""
+    with _lock:
+        return _best.get((kind, stage, _bucket(flops)))
+
+
 def tune(plan, stage: int, run, flops: float, kind: str):
     """Choose the tiling of one GEMM stage of ``plan``; ``run()`` launches that
     stage alone on the current stream (its inputs already in place)."""

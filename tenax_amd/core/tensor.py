@@ -465,8 +465,10 @@ def _graded(t: "SymmetricTensor") -> bool:
 def _block_sign_vector(t: "SymmetricTensor", negative) -> torch.Tensor:
     """+-1 per element of t's packed data, -1 on the blocks flagged in ``negative``."""
     sizes = np.prod(t._shapes, axis=1).astype(np.int64) if t.n_blocks else np.zeros(0, np.int64)
-    per = np.repeat(np.where(np.asarray(negative, bool), -1.0, 1.0), sizes)
-    return torch.from_numpy(per).to(t._data.device)
+    sign = np.where(np.asarray(negative, bool), -1.0, 1.0)
+    dev = t._data.device
+    return torch.repeat_interleave(torch.from_numpy(sign).to(dev), torch.from_numpy(sizes).to(dev),
+                                   output_size=int(sizes.sum()))
 
 
 def _transpose_raw(t: "SymmetricTensor", axes, graded=None) -> "SymmetricTensor":
@@ -526,10 +528,19 @@ def _dense_gather_index(indices, keys, shapes) -> torch.Tensor:
     return torch.from_numpy(idx).to(_device())
 
 
-def _block_ranges_index(offsets, shapes, order) -> np.ndarray:
-    sizes = shapes.prod(axis=1).astype(np.int64)
-    return np.concatenate([np.arange(offsets[i], offsets[i] + sizes[i], dtype=np.int64) for i in order]) \
-        if len(order) else np.zeros(0, np.int64)
+def _block_ranges_index(offsets, shapes, order, device) -> torch.Tensor:
+    """Concatenated element ranges of blocks ``order`` (in that order) as a
+    device index, built on the device from per-block starts and sizes (two
+    small host arrays cross PCIe, not one index per element)."""
+    order = np.asarray(order, np.int64)
+    if len(order) == 0:
+        return torch.zeros(0, dtype=torch.int64, device=device)
+    sz = shapes[order].prod(axis=1).astype(np.int64)
+    shift = np.asarray(offsets, np.int64)[order] - (np.cumsum(sz) - sz)
+    total = int(sz.sum())
+    shift_d = torch.repeat_interleave(torch.from_numpy(shift).to(device), torch.from_numpy(sz).to(device),
+                                      output_size=total)
+    return torch.arange(total, dtype=torch.int64, device=device) + shift_d
 
 
 def _reorder(t: SymmetricTensor, indices, new_keys, conj=False) -> SymmetricTensor:
@@ -537,7 +548,7 @@ def _reorder(t: SymmetricTensor, indices, new_keys, conj=False) -> SymmetricTens
     keys = new_keys[order]
     shapes = t._shapes[order]
     offs, tot = SymmetricTensor._layout(shapes)
-    gi = torch.from_numpy(_block_ranges_index(t._offsets, t._shapes, order)).to(t._data.device)
+    gi = _block_ranges_index(t._offsets, t._shapes, order, t._data.device)
     data = t._data[gi] if tot else t._data[:0]
     if conj:
         data = data.conj()
@@ -560,8 +571,8 @@ def _union_add(a: SymmetricTensor, b: SymmetricTensor) -> SymmetricTensor:
         src_order = [kx[allk[i]] for i in dst_order]
         if not dst_order:
             continue
-        di = torch.from_numpy(_block_ranges_index(offs, shapes, dst_order)).to(out.device)
-        si = torch.from_numpy(_block_ranges_index(src._offsets, src._shapes, src_order)).to(out.device)
+        di = _block_ranges_index(offs, shapes, dst_order, out.device)
+        si = _block_ranges_index(src._offsets, src._shapes, src_order, out.device)
         out.index_add_(0, di, src._data[si])
     return SymmetricTensor._wrap(a._indices, keys, shapes, offs, out)
 
@@ -578,7 +589,7 @@ def inner(a: Tensor, b: Tensor):
         common = [k for k in ka if k in kb]
         if not common:
             return torch.zeros((), dtype=torch.float64, device=a._data.device)
-        ia = torch.from_numpy(_block_ranges_index(a._offsets, a._shapes, [ka[k] for k in common])).to(a._data.device)
-        ib = torch.from_numpy(_block_ranges_index(b._offsets, b._shapes, [kb[k] for k in common])).to(a._data.device)
+        ia = _block_ranges_index(a._offsets, a._shapes, [ka[k] for k in common], a._data.device)
+        ib = _block_ranges_index(b._offsets, b._shapes, [kb[k] for k in common], a._data.device)
         return torch.dot(a._data[ia].conj(), b._data[ib])
     return torch.sum(a.todense().conj() * b.todense())
