@@ -384,6 +384,10 @@ class LanczosGraph:
             self.ptr = None
 
     def __del__(self):
+        import sys
+
+        if sys.is_finalizing():  # the HIP runtime may already be gone: leave it to process exit
+            return
         try:
             self.close()
         except Exception:

@@ -267,8 +267,9 @@ __device__ __forceinline__ int mix_nj(int D0, int D1, int D2, int ca) {
 // list, MIX_BATCH X loads are issued before their products, so a wave waits
 // on memory once per batch rather than once per term.
 constexpr int MIX_BATCH = 8;
+constexpr int MIX_WG = 256;  // threads per mix workgroup = (u, v) pairs per task
 
-__global__ void __launch_bounds__(256) mix_kernel(const MixItem *__restrict__ items,
+__global__ void __launch_bounds__(MIX_WG) mix_kernel(const MixItem *__restrict__ items,
                                                   const int32_t *__restrict__ task_item,
                                                   const int64_t *__restrict__ task_start,
                                                   const int32_t *__restrict__ item_entry,
@@ -280,7 +281,7 @@ __global__ void __launch_bounds__(256) mix_kernel(const MixItem *__restrict__ it
   const int D0 = ip->D0, D1 = ip->D1, D2 = ip->D2, ca = ip->ca;
   const int nj = mix_nj(D0, D1, D2, ca);
   const int64_t nuv = (int64_t)D0 * D1 * D2 / nj;
-  const int64_t nuvp = (nuv + 255) / 256 * 256;
+  const int64_t nuvp = (nuv + MIX_WG - 1) / MIX_WG * MIX_WG;
   const int64_t start = task_start[blockIdx.x];
   const int jc = (int)(start / nuvp);
   const int j0 = jc * MIX_NJ;
@@ -579,7 +580,7 @@ __global__ void mix_task_count_kernel(int64_t ni, const MixItem *__restrict__ it
   const int nj = mix_nj(ip->D0, ip->D1, ip->D2, ip->ca);
   const int64_t nuv = nj ? (int64_t)ip->D0 * ip->D1 * ip->D2 / nj : 0;
   // workgroups of 256 (u, v) pairs, per chunk of MIX_NJ coefficient indices
-  cnt[k] = (nj + MIX_NJ - 1) / MIX_NJ * ((nuv + 255) / 256);
+  cnt[k] = (nj + MIX_NJ - 1) / MIX_NJ * ((nuv + MIX_WG - 1) / MIX_WG);
 }
 
 __global__ void mix_task_write_kernel(int64_t ni, const int64_t *__restrict__ cnt, const int64_t *__restrict__ off,
@@ -588,7 +589,7 @@ __global__ void mix_task_write_kernel(int64_t ni, const int64_t *__restrict__ cn
   if (k >= ni) return;
   for (int64_t c = 0; c < cnt[k]; ++c) {
     titem[off[k] + c] = (int32_t)k;
-    tstart[off[k] + c] = c * 256;
+    tstart[off[k] + c] = c * MIX_WG;
   }
 }
 
@@ -697,7 +698,7 @@ static void order_mix_tasks(FusedPlan &P, int64_t nI, hipStream_t s) {
   ts.reserve(P.n_mix_tasks);
   for (const auto &g : groups) {
     const int64_t nuv = nuv_of(items[g[0]]);
-    const int64_t per = (nuv + 255) / 256, nuvp = per * 256;
+    const int64_t per = (nuv + MIX_WG - 1) / MIX_WG, nuvp = per * MIX_WG;
     for (int64_t sl = 0; sl < per; ++sl)
       for (int64_t i : g) {
         const MixItem &it = items[i];
@@ -705,7 +706,7 @@ static void order_mix_tasks(FusedPlan &P, int64_t nI, hipStream_t s) {
         if (nj == 0 || nuv_of(it) == 0) continue;
         for (int jc = 0; jc < (nj + MIX_NJ - 1) / MIX_NJ; ++jc) {
           ti.push_back((int32_t)i);
-          ts.push_back((int64_t)jc * nuvp + sl * 256);
+          ts.push_back((int64_t)jc * nuvp + sl * MIX_WG);
         }
       }
   }
@@ -974,7 +975,7 @@ static void run_fused(const FusedPlan &P, const double *s1A, const double *s1B, 
     hipLaunchKernelGGL(coef_kernel, dim3(nb256(P.n_terms)), dim3(256), 0, s, P.n_terms, P.cspec.p, W1, W2,
                        P.coef.p);
   launch_grouped_gemm(P.s1, s1A, s1B, P.X.p, compute, false, s);
-  hipLaunchKernelGGL(mix_kernel, dim3(P.n_mixf_tasks), dim3(256), 0, s, P.items.p, P.mixf_titem.p, P.mixf_tstart.p,
+  hipLaunchKernelGGL(mix_kernel, dim3(P.n_mixf_tasks), dim3(MIX_WG), 0, s, P.items.p, P.mixf_titem.p, P.mixf_tstart.p,
                      P.item_entry.p, P.ents.p, P.coef.p, P.X.p, P.Y.p);
   launch_grouped_gemm(P.s2, P.Y.p, s2B, out, compute, false, s);
   TNX_CHECK_HIP(hipGetLastError());
@@ -1037,7 +1038,7 @@ static void run_fused_perblock(FusedPlan &P, const double *s1A, const double *s1
     launch_prepared(P.s1.a_lay, P.s1.b_lay, compute, P.pb1_cnt[x], s, P.s1.tasks.p + P.pb1_off[x], P.s1.outs.p,
                     P.s1.pairs.p, s1A, s1B, P.X.p, false);
   for (int64_t k = 0; k < P.n_items; ++k)
-    hipLaunchKernelGGL(mix_kernel, dim3(P.pbm_cnt[k]), dim3(256), 0, s, P.items.p, P.mix_titem.p + P.pbm_off[k],
+    hipLaunchKernelGGL(mix_kernel, dim3(P.pbm_cnt[k]), dim3(MIX_WG), 0, s, P.items.p, P.mix_titem.p + P.pbm_off[k],
                        P.mix_tstart.p + P.pbm_off[k], P.item_entry.p, P.ents.p, P.coef.p, P.X.p, P.Y.p);
   for (const auto &z : P.pb2_zero) {
     const int64_t n = (int64_t)z.M * z.N;
@@ -1529,7 +1530,7 @@ int tnx_matvec_execute_stage(const tnx_plan *plan, int32_t stage, const double *
         launch_grouped_gemm(P.s1, L, x, P.X.p, compute, false, s, !gemm_only);
         break;
       case 2:
-        hipLaunchKernelGGL(mix_kernel, dim3(P.n_mixf_tasks), dim3(256), 0, s, P.items.p, P.mixf_titem.p,
+        hipLaunchKernelGGL(mix_kernel, dim3(P.n_mixf_tasks), dim3(MIX_WG), 0, s, P.items.p, P.mixf_titem.p,
                            P.mixf_tstart.p, P.item_entry.p, P.ents.p, P.coef.p, P.X.p, P.Y.p);
         break;
       case 3:
