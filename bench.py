@@ -98,7 +98,8 @@ def stage_kernel(prob, stage):
     t = prob.plan.tiling(stage)
     bm, bn = TILE_SHAPES[t["shape"]]
     lay = 1 if stage == 1 else 0  # stage 1 reads L as K x M, stage 3 reads Y as M x K
-    return f"grouped_gemm_kernel<{lay},0,{bm},{bn}>", t["n_tasks"], t
+    # last template argument: fermionic signs (the Heisenberg chain is bosonic)
+    return f"grouped_gemm_kernel<{lay},0,{bm},{bn},false>", t["n_tasks"], t
 
 
 def time_events(fn, reps):

@@ -23,9 +23,7 @@
 
 namespace tnx {
 
-void launch_prepared(int a_lay, int b_lay, int compute, int n_tasks, hipStream_t s, const GemmTask *tasks,
-                     const GemmOut *outs, const GemmPair *pairs, const double *A, const double *B, double *C,
-                     bool accumulate);
+
 
 static unsigned nb256(int64_t n) { return (unsigned)((n + 255) / 256); }
 
@@ -1035,7 +1033,7 @@ static void run_fused_perblock(FusedPlan &P, const double *s1A, const double *s1
   }
   hipLaunchKernelGGL(coef_kernel, dim3(nb256(P.n_terms)), dim3(256), 0, s, P.n_terms, P.cspec.p, W1, W2, P.coef.p);
   for (int x = 0; x < P.s1.n_outs; ++x)
-    launch_prepared(P.s1.a_lay, P.s1.b_lay, compute, P.pb1_cnt[x], s, P.s1.tasks.p + P.pb1_off[x], P.s1.outs.p,
+    launch_prepared(P.s1.a_lay, P.s1.b_lay, P.s1.signs, compute, P.pb1_cnt[x], s, P.s1.tasks.p + P.pb1_off[x], P.s1.outs.p,
                     P.s1.pairs.p, s1A, s1B, P.X.p, false);
   for (int64_t k = 0; k < P.n_items; ++k)
     hipLaunchKernelGGL(mix_kernel, dim3(P.pbm_cnt[k]), dim3(MIX_WG), 0, s, P.items.p, P.mix_titem.p + P.pbm_off[k],
@@ -1045,7 +1043,7 @@ static void run_fused_perblock(FusedPlan &P, const double *s1A, const double *s1
     if (n) hipLaunchKernelGGL(zero_kernel, dim3(nb256(n)), dim3(256), 0, s, n, out + z.c_off);
   }
   for (int k = 0; k < P.s2.n_pairs; ++k)
-    launch_prepared(P.s2.a_lay, P.s2.b_lay, compute, P.pb2_cnt[k], s, P.pb2_tasks.p + P.pb2_off[k], P.pb2_outs.p,
+    launch_prepared(P.s2.a_lay, P.s2.b_lay, P.s2.signs, compute, P.pb2_cnt[k], s, P.pb2_tasks.p + P.pb2_off[k], P.pb2_outs.p,
                     P.s2.pairs.p, P.Y.p, s2B, out, !P.pb2_first[k]);
   TNX_CHECK_HIP(hipGetLastError());
 }

@@ -167,13 +167,14 @@ __device__ __forceinline__ int b_lds(int tid, int q, int i) {
   return (tid / BN + RPP * (4 * q + i)) * BP + tid % BN;
 }
 
-template <int ALAY, int BM, int BN, class T>
+// SIGNED: the program has pairs with a fermionic sign (otherwise no sign work).
+template <int ALAY, int BM, int BN, bool SIGNED, class T>
 __device__ __forceinline__ void store_tiles(T *As, T *Bs, const Stage<BM, BN> &st, int tid) {
 #pragma unroll
   for (int q = 0; q < BM / 64; ++q)
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      As[a_lds<ALAY, BM>(tid, q, i)] = (T)signed_a(st.a[q][i], st.neg_hi);
+      As[a_lds<ALAY, BM>(tid, q, i)] = (T)(SIGNED ? signed_a(st.a[q][i], st.neg_hi) : st.a[q][i]);
 #pragma unroll
   for (int q = 0; q < BN / 64; ++q)
 #pragma unroll
@@ -182,14 +183,14 @@ __device__ __forceinline__ void store_tiles(T *As, T *Bs, const Stage<BM, BN> &s
 }
 
 // Split-FP32 staging: hi = fp32(x), lo = fp32(x - hi).
-template <int ALAY, int BM, int BN>
+template <int ALAY, int BM, int BN, bool SIGNED>
 __device__ __forceinline__ void store_tiles_split(float *Ah, float *Al, float *Bh, float *Bl,
                                                   const Stage<BM, BN> &st, int tid) {
 #pragma unroll
   for (int q = 0; q < BM / 64; ++q)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const double x = signed_a(st.a[q][i], st.neg_hi);
+      const double x = SIGNED ? signed_a(st.a[q][i], st.neg_hi) : st.a[q][i];
       const float h = (float)x;
       const int at = a_lds<ALAY, BM>(tid, q, i);
       Ah[at] = h;
@@ -243,7 +244,7 @@ __device__ __forceinline__ void store_partial(double *__restrict__ W, int row0, 
 // (BM/32) x (BN/32) MFMA 16x16 blocks.
 // At least two waves per SIMD (the 128-wide tiles held to 256 registers),
 // except split FP32 at 128x128, whose hi/lo operands would then spill.
-template <int ALAY, int COMPUTE, int BM, int BN>
+template <int ALAY, int COMPUTE, int BM, int BN, bool SIGNED>
 __global__ __launch_bounds__(GEMM_THREADS) __attribute__((amdgpu_waves_per_eu(
     (COMPUTE == 2 && BM * BN > 8192) ? 1 : 2))) void grouped_gemm_kernel(
     const GemmTask *__restrict__ tasks, const GemmOut *__restrict__ outs,
@@ -286,10 +287,10 @@ __global__ __launch_bounds__(GEMM_THREADS) __attribute__((amdgpu_waves_per_eu(
 
   auto stage_store = [&](int buf, const Stage<BM, BN> &st) {
     if (COMPUTE == 2)
-      store_tiles_split<ALAY, BM, BN>((float *)As[buf][0], (float *)As[buf][NT - 1], (float *)Bs[buf][0],
+      store_tiles_split<ALAY, BM, BN, SIGNED>((float *)As[buf][0], (float *)As[buf][NT - 1], (float *)Bs[buf][0],
                                       (float *)Bs[buf][NT - 1], st, tid);
     else
-      store_tiles<ALAY, BM, BN, T>(As[buf][0], Bs[buf][0], st, tid);
+      store_tiles<ALAY, BM, BN, SIGNED, T>(As[buf][0], Bs[buf][0], st, tid);
   };
 
   // One BK-deep tile from LDS stage `buf`: the operands of k-step kk+1 are
@@ -483,20 +484,20 @@ __global__ __launch_bounds__(256) void split_reduce_kernel(const SplitTile *__re
   }
 }
 
-template <int ALAY, int BM, int BN>
+template <int ALAY, int BM, int BN, bool SIGNED>
 static void launch_tile(int compute, dim3 grid, hipStream_t s, const GemmTask *tasks, const GemmOut *outs,
                         const GemmPair *pairs, const double *A, const double *B, double *C, double *ws, int acc) {
   switch (compute) {
     case TNX_COMPUTE_F64:
-      hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, 0, BM, BN>), grid, dim3(GEMM_THREADS), 0, s, tasks, outs, pairs,
+      hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, 0, BM, BN, SIGNED>), grid, dim3(GEMM_THREADS), 0, s, tasks, outs, pairs,
                          A, B, C, ws, acc);
       break;
     case TNX_COMPUTE_F32:
-      hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, 1, BM, BN>), grid, dim3(GEMM_THREADS), 0, s, tasks, outs, pairs,
+      hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, 1, BM, BN, SIGNED>), grid, dim3(GEMM_THREADS), 0, s, tasks, outs, pairs,
                          A, B, C, ws, acc);
       break;
     case TNX_COMPUTE_SPLIT_F32:
-      hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, 2, BM, BN>), grid, dim3(GEMM_THREADS), 0, s, tasks, outs, pairs,
+      hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, 2, BM, BN, SIGNED>), grid, dim3(GEMM_THREADS), 0, s, tasks, outs, pairs,
                          A, B, C, ws, acc);
       break;
     default:
@@ -504,29 +505,36 @@ static void launch_tile(int compute, dim3 grid, hipStream_t s, const GemmTask *t
   }
 }
 
-template <int ALAY>
+template <int ALAY, bool SIGNED>
 static void launch_shape(int shape, int compute, dim3 grid, hipStream_t s, const GemmTask *tasks,
                          const GemmOut *outs, const GemmPair *pairs, const double *A, const double *B, double *C,
                          double *ws, int acc) {
   switch (shape) {
-    case 0: launch_tile<ALAY, 64, 64>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc); break;
-    case 1: launch_tile<ALAY, 128, 64>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc); break;
-    case 2: launch_tile<ALAY, 64, 128>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc); break;
-    case 3: launch_tile<ALAY, 128, 128>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc); break;
+    case 0: launch_tile<ALAY, 64, 64, SIGNED>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc); break;
+    case 1: launch_tile<ALAY, 128, 64, SIGNED>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc); break;
+    case 2: launch_tile<ALAY, 64, 128, SIGNED>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc); break;
+    case 3: launch_tile<ALAY, 128, 128, SIGNED>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc); break;
     default: throw Error{TNX_ERR_ARG, "unknown tile shape " + std::to_string(shape)};
   }
 }
 
-static void launch_any(int a_lay, int b_lay, int shape, int compute, int n_tasks, hipStream_t s,
+static void launch_any(int a_lay, int b_lay, bool signs, int shape, int compute, int n_tasks, hipStream_t s,
                        const GemmTask *tasks, const GemmOut *outs, const GemmPair *pairs, const double *A,
                        const double *B, double *C, double *ws, int acc) {
   if (n_tasks <= 0) return;
   TNX_REQUIRE(b_lay == LAY_KN, TNX_ERR_ARG, "B operands must be K x N row-major");
   dim3 grid(n_tasks);
-  if (a_lay == LAY_MK)
-    launch_shape<LAY_MK>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc);
-  else
-    launch_shape<LAY_KM>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc);
+  if (a_lay == LAY_MK) {
+    if (signs)
+      launch_shape<LAY_MK, true>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc);
+    else
+      launch_shape<LAY_MK, false>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc);
+  } else {
+    if (signs)
+      launch_shape<LAY_KM, true>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc);
+    else
+      launch_shape<LAY_KM, false>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc);
+  }
   TNX_CHECK_HIP(hipGetLastError());
 }
 
@@ -534,11 +542,11 @@ void launch_grouped_gemm(const GemmProgram &prog, const double *A, const double 
                          bool accumulate, hipStream_t s, bool reduce) {
   const int acc = accumulate ? 1 : 0;
   if (!prog.tiled) {
-    launch_any(prog.a_lay, prog.b_lay, 0, compute, prog.n_tasks, s, prog.tasks.p, prog.outs.p, prog.pairs.p, A, B,
+    launch_any(prog.a_lay, prog.b_lay, prog.signs, 0, compute, prog.n_tasks, s, prog.tasks.p, prog.outs.p, prog.pairs.p, A, B,
                C, nullptr, acc);
     return;
   }
-  launch_any(prog.a_lay, prog.b_lay, prog.cfg.shape, compute, prog.n_vtasks, s, prog.vtasks.p, prog.outs.p,
+  launch_any(prog.a_lay, prog.b_lay, prog.signs, prog.cfg.shape, compute, prog.n_vtasks, s, prog.vtasks.p, prog.outs.p,
              prog.pairs.p, A, B, C, prog.ws.p, acc);
   if (reduce && prog.n_red > 0) {
     const dim3 g(prog.n_red), b(256);
@@ -553,10 +561,10 @@ void launch_grouped_gemm(const GemmProgram &prog, const double *A, const double 
 }
 
 // Per-block path helper: launch a prepared single-pair program (64x64 whole tiles).
-void launch_prepared(int a_lay, int b_lay, int compute, int n_tasks, hipStream_t s, const GemmTask *tasks,
-                     const GemmOut *outs, const GemmPair *pairs, const double *A, const double *B, double *C,
-                     bool accumulate) {
-  launch_any(a_lay, b_lay, 0, compute, n_tasks, s, tasks, outs, pairs, A, B, C, nullptr, accumulate ? 1 : 0);
+void launch_prepared(int a_lay, int b_lay, bool signs, int compute, int n_tasks, hipStream_t s,
+                     const GemmTask *tasks, const GemmOut *outs, const GemmPair *pairs, const double *A,
+                     const double *B, double *C, bool accumulate) {
+  launch_any(a_lay, b_lay, signs, 0, compute, n_tasks, s, tasks, outs, pairs, A, B, C, nullptr, accumulate ? 1 : 0);
 }
 
 }  // namespace tnx

@@ -156,6 +156,7 @@ constexpr int GEMM_THREADS = 256;
 // Device-resident grouped-GEMM program: the work lists of one launch.
 struct GemmProgram {
   int a_lay = LAY_MK, b_lay = LAY_KN;
+  bool signs = false;  // some pair carries a fermionic sign (set when finalized)
   DevBuf<GemmPair> pairs;
   DevBuf<GemmOut> outs;
   DevBuf<GemmTask> tasks;
@@ -188,9 +189,9 @@ void set_tiling(GemmProgram &prog, const TileCfg &cfg, hipStream_t s);
 void launch_grouped_gemm(const GemmProgram &prog, const double *A, const double *B, double *C,
                          int compute, bool accumulate, hipStream_t s, bool reduce = true);
 // Launch a prepared program slice (tasks/outs/pairs already on the device).
-void launch_prepared(int a_lay, int b_lay, int compute, int n_tasks, hipStream_t s, const GemmTask *tasks,
-                     const GemmOut *outs, const GemmPair *pairs, const double *A, const double *B, double *C,
-                     bool accumulate);
+void launch_prepared(int a_lay, int b_lay, bool signs, int compute, int n_tasks, hipStream_t s,
+                     const GemmTask *tasks, const GemmOut *outs, const GemmPair *pairs, const double *A,
+                     const double *B, double *C, bool accumulate);
 
 // Device-side tiling of a program whose pairs/outs are already on the device:
 // builds the tile list (longest K first) and downloads host mirrors.

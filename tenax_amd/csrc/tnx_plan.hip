@@ -468,6 +468,9 @@ void finalize_program(GemmProgram &prog, hipStream_t s) {
   prog.pairs.download(prog.h_pairs.data(), prog.n_pairs, s);
   TNX_CHECK_HIP(hipStreamSynchronize(s));
   check_operand_extents(prog);
+  prog.signs = false;
+  for (const GemmPair &p : prog.h_pairs)
+    if (p.neg) prog.signs = true;
   double f = 0;
   for (const auto &o : prog.h_outs)
     for (int k = o.pair_begin; k < o.pair_end; ++k) f += 2.0 * o.M * (double)o.N * prog.h_pairs[k].K;
@@ -1047,7 +1050,7 @@ int tnx_contract_execute_perblock(const tnx_plan *plan, const double *a, const d
     ensure_perblock(P, s);
     TNX_REQUIRE(a && b && c || P.prog.n_pairs == 0, TNX_ERR_ARG, "null buffer");
     for (int k = 0; k < P.prog.n_pairs; ++k)
-      launch_prepared(P.prog.a_lay, P.prog.b_lay, compute, P.pb_task_cnt[k], s, P.pb_tasks.p + P.pb_task_off[k],
+      launch_prepared(P.prog.a_lay, P.prog.b_lay, P.prog.signs, compute, P.pb_task_cnt[k], s, P.pb_tasks.p + P.pb_task_off[k],
                       P.pb_outs.p, P.prog.pairs.p, a, b, c, !P.pb_first[k]);
   });
 }
