@@ -262,10 +262,15 @@ __device__ __forceinline__ int mix_nj(int D0, int D1, int D2, int ca) {
 // Item fields are read one by one (uniform across the workgroup, so they go
 // through scalar loads; a copied struct whose field is picked by a run-time
 // axis ends up in LDS).  The item's (term, b) products come as a flat entry
-// list, MIX_BATCH X loads are issued before their products, so a wave waits
-// on memory once per batch rather than once per term.
-constexpr int MIX_BATCH = 8;
-constexpr int MIX_WG = 256;  // threads per mix workgroup = (u, v) pairs per task
+// list; each thread covers MIX_UV (u, v) positions MIX_WG apart (coalesced
+// per position), so one entry's scalar loads (offsets, strides,
+// coefficients) serve MIX_UV X loads, and MIX_BATCH entries' loads are issued
+// before their products: a wave waits on memory once per batch, and the
+// chain of dependent descriptor loads is paid once per MIX_UV positions.
+constexpr int MIX_BATCH = 4;
+constexpr int MIX_WG = 256;                // threads per mix workgroup
+constexpr int MIX_UV = 4;                  // (u, v) positions per thread
+constexpr int MIX_SPAN = MIX_WG * MIX_UV;  // (u, v) positions per task
 
 __global__ void __launch_bounds__(MIX_WG) mix_kernel(const MixItem *__restrict__ items,
                                                   const int32_t *__restrict__ task_item,
@@ -279,54 +284,74 @@ __global__ void __launch_bounds__(MIX_WG) mix_kernel(const MixItem *__restrict__
   const int D0 = ip->D0, D1 = ip->D1, D2 = ip->D2, ca = ip->ca;
   const int nj = mix_nj(D0, D1, D2, ca);
   const int64_t nuv = (int64_t)D0 * D1 * D2 / nj;
-  const int64_t nuvp = (nuv + MIX_WG - 1) / MIX_WG * MIX_WG;
+  const int64_t nuvp = (nuv + MIX_SPAN - 1) / MIX_SPAN * MIX_SPAN;
   const int64_t start = task_start[blockIdx.x];
   const int jc = (int)(start / nuvp);
   const int j0 = jc * MIX_NJ;
-  const int64_t uv = start - (int64_t)jc * nuvp + threadIdx.x;
-  if (uv >= nuv) return;
-  // (u, v) = the two non-coefficient indices in storage order
+  const int64_t uv0 = start - (int64_t)jc * nuvp + threadIdx.x;
+  if (uv0 >= nuv) return;
+  // (u, v) = the two non-coefficient indices in storage order.  A position
+  // past the item's end computes position uv0 again and is not stored.
   const int dv = ca == 2 ? D1 : D2;
-  int u, v;
-  if (nuv < (int64_t(1) << 31)) {
-    u = (int)((uint32_t)uv / (uint32_t)dv);
-    v = (int)uv - u * dv;
-  } else {
-    u = (int)(uv / dv);
-    v = (int)(uv - (int64_t)u * dv);
+  int us[MIX_UV], vs[MIX_UV];
+  bool ok[MIX_UV];
+#pragma unroll
+  for (int r = 0; r < MIX_UV; ++r) {
+    const int64_t uv = uv0 + (int64_t)r * MIX_WG;
+    ok[r] = uv < nuv;
+    const int64_t w = ok[r] ? uv : uv0;
+    if (nuv < (int64_t(1) << 31)) {
+      us[r] = (int)((uint32_t)w / (uint32_t)dv);
+      vs[r] = (int)w - us[r] * dv;
+    } else {
+      us[r] = (int)(w / dv);
+      vs[r] = (int)(w - (int64_t)us[r] * dv);
+    }
   }
-  // four named accumulators: a private array would be placed in LDS
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  double acc[MIX_UV][MIX_NJ];
+#pragma unroll
+  for (int r = 0; r < MIX_UV; ++r)
+#pragma unroll
+    for (int q = 0; q < MIX_NJ; ++q) acc[r][q] = 0.0;
   const int nq = min(MIX_NJ, nj - j0);
   const int e1 = item_entry[item + 1];
   for (int e = item_entry[item]; e < e1; e += MIX_BATCH) {
-    double x[MIX_BATCH];
+    double x[MIX_BATCH][MIX_UV];
 #pragma unroll
     for (int k = 0; k < MIX_BATCH; ++k)
       if (e + k < e1) {
         const MixEntry *mp = ents + e + k;
-        x[k] = X[mp->x_off + (int64_t)u * mp->su + (int64_t)v * mp->sv];
+        const int64_t xo = mp->x_off, su = mp->su, sv = mp->sv;
+#pragma unroll
+        for (int r = 0; r < MIX_UV; ++r) x[k][r] = X[xo + us[r] * su + vs[r] * sv];
       }
 #pragma unroll
     for (int k = 0; k < MIX_BATCH; ++k)
       if (e + k < e1) {
         const double *c = coef + ents[e + k].coef_off + j0;
-        a0 += c[0] * x[k];
-        if (nq > 1) a1 += c[1] * x[k];
-        if (nq > 2) a2 += c[2] * x[k];
-        if (nq > 3) a3 += c[3] * x[k];
+        double cq[MIX_NJ];
+#pragma unroll
+        for (int q = 0; q < MIX_NJ; ++q) cq[q] = q < nq ? c[q] : 0.0;
+#pragma unroll
+        for (int r = 0; r < MIX_UV; ++r)
+#pragma unroll
+          for (int q = 0; q < MIX_NJ; ++q) acc[r][q] += cq[q] * x[k][r];
       }
   }
   // element offset of (j, u, v) in the item's row-major (D0, D1, D2) block
-  int64_t base, sj;
-  if (ca == 0) { base = uv; sj = nuv; }
-  else if (ca == 1) { base = (int64_t)u * D1 * D2 + v; sj = D2; }
-  else { base = uv * D2; sj = 1; }
-  double *yp = Y + ip->y_off + base + (int64_t)j0 * sj;
-  yp[0] = a0;
-  if (nq > 1) yp[sj] = a1;
-  if (nq > 2) yp[2 * sj] = a2;
-  if (nq > 3) yp[3 * sj] = a3;
+#pragma unroll
+  for (int r = 0; r < MIX_UV; ++r) {
+    if (!ok[r]) continue;
+    const int64_t uv = uv0 + (int64_t)r * MIX_WG;
+    int64_t base, sj;
+    if (ca == 0) { base = uv; sj = nuv; }
+    else if (ca == 1) { base = (int64_t)us[r] * D1 * D2 + vs[r]; sj = D2; }
+    else { base = uv * D2; sj = 1; }
+    double *yp = Y + ip->y_off + base + (int64_t)j0 * sj;
+#pragma unroll
+    for (int q = 0; q < MIX_NJ; ++q)
+      if (q < nq) yp[q * sj] = acc[r][q];
+  }
 }
 
 // ------------------------------------------------------------------ plan-building kernels
@@ -577,8 +602,8 @@ __global__ void mix_task_count_kernel(int64_t ni, const MixItem *__restrict__ it
   const MixItem *ip = items + k;
   const int nj = mix_nj(ip->D0, ip->D1, ip->D2, ip->ca);
   const int64_t nuv = nj ? (int64_t)ip->D0 * ip->D1 * ip->D2 / nj : 0;
-  // workgroups of 256 (u, v) pairs, per chunk of MIX_NJ coefficient indices
-  cnt[k] = (nj + MIX_NJ - 1) / MIX_NJ * ((nuv + MIX_WG - 1) / MIX_WG);
+  // one task per MIX_SPAN (u, v) positions and chunk of MIX_NJ coefficient indices
+  cnt[k] = (nj + MIX_NJ - 1) / MIX_NJ * ((nuv + MIX_SPAN - 1) / MIX_SPAN);
 }
 
 __global__ void mix_task_write_kernel(int64_t ni, const int64_t *__restrict__ cnt, const int64_t *__restrict__ off,
@@ -587,7 +612,7 @@ __global__ void mix_task_write_kernel(int64_t ni, const int64_t *__restrict__ cn
   if (k >= ni) return;
   for (int64_t c = 0; c < cnt[k]; ++c) {
     titem[off[k] + c] = (int32_t)k;
-    tstart[off[k] + c] = c * MIX_WG;
+    tstart[off[k] + c] = c * MIX_SPAN;
   }
 }
 
@@ -696,7 +721,7 @@ static void order_mix_tasks(FusedPlan &P, int64_t nI, hipStream_t s) {
   ts.reserve(P.n_mix_tasks);
   for (const auto &g : groups) {
     const int64_t nuv = nuv_of(items[g[0]]);
-    const int64_t per = (nuv + MIX_WG - 1) / MIX_WG, nuvp = per * MIX_WG;
+    const int64_t per = (nuv + MIX_SPAN - 1) / MIX_SPAN, nuvp = per * MIX_SPAN;
     for (int64_t sl = 0; sl < per; ++sl)
       for (int64_t i : g) {
         const MixItem &it = items[i];
@@ -704,7 +729,7 @@ static void order_mix_tasks(FusedPlan &P, int64_t nI, hipStream_t s) {
         if (nj == 0 || nuv_of(it) == 0) continue;
         for (int jc = 0; jc < (nj + MIX_NJ - 1) / MIX_NJ; ++jc) {
           ti.push_back((int32_t)i);
-          ts.push_back((int64_t)jc * nuvp + sl * MIX_WG);
+          ts.push_back((int64_t)jc * nuvp + sl * MIX_SPAN);
         }
       }
   }
