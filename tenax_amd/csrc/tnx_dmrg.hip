@@ -230,22 +230,32 @@ struct CoefSpec {
   int32_t kind, nb, nmid, nj;
 };
 
-__global__ void coef_kernel(int64_t n, const CoefSpec *__restrict__ cs, const double *__restrict__ W1,
-                            const double *__restrict__ W2, double *__restrict__ coef) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const CoefSpec c = cs[t];
-  for (int b = 0; b < c.nb; ++b)
-    for (int j = 0; j < c.nj; ++j) {
-      double v;
-      if (c.kind == 0) {  // sum_e Wl[b,e] Wr[e,f]
-        v = 0.0;
-        for (int e = 0; e < c.nmid; ++e) v += W1[c.w1_off + (int64_t)b * c.nmid + e] * W2[c.w2_off + (int64_t)e * c.nj + j];
-      } else {  // W[b,0,0,e]
-        v = W1[c.w1_off + (int64_t)b * c.nj + j];
-      }
-      coef[c.coef_off + (int64_t)b * c.nj + j] = v;
+// One workgroup per term, one thread per (b, j) coefficient: only the short
+// sum over the MPO bond in between is sequential.
+constexpr int COEF_WG = 64;
+
+__global__ void __launch_bounds__(COEF_WG) coef_kernel(const CoefSpec *__restrict__ cs, const double *__restrict__ W1,
+                                                       const double *__restrict__ W2, double *__restrict__ coef) {
+  const CoefSpec c = cs[blockIdx.x];
+  const int nbj = c.nb * c.nj;
+  for (int t = threadIdx.x; t < nbj; t += COEF_WG) {
+    const int b = t / c.nj, j = t - b * c.nj;
+    double v;
+    if (c.kind == 0) {  // sum_e Wl[b,e] Wr[e,f]
+      v = 0.0;
+      for (int e = 0; e < c.nmid; ++e) v += W1[c.w1_off + (int64_t)b * c.nmid + e] * W2[c.w2_off + (int64_t)e * c.nj + j];
+    } else {  // W[b,0,0,e]
+      v = W1[c.w1_off + (int64_t)b * c.nj + j];
     }
+    coef[c.coef_off + t] = v;
+  }
+}
+
+static void launch_coef(int64_t n_terms, const CoefSpec *cs, const double *W1, const double *W2, double *coef,
+                        hipStream_t s) {
+  if (n_terms <= 0) return;
+  TNX_REQUIRE(n_terms < (int64_t(1) << 31), TNX_ERR_ARG, "too many MPO terms for one coefficient launch");
+  hipLaunchKernelGGL(coef_kernel, dim3((uint32_t)n_terms), dim3(COEF_WG), 0, s, cs, W1, W2, coef);
 }
 
 // Y[.., j, ..] = sum_terms sum_b coef[b, j] X_t[b, u, v] with j on axis ca.
@@ -995,8 +1005,7 @@ static void run_fused(const FusedPlan &P, const double *s1A, const double *s1B, 
     return;
   }
   if (!coef_ready)
-    hipLaunchKernelGGL(coef_kernel, dim3(nb256(P.n_terms)), dim3(256), 0, s, P.n_terms, P.cspec.p, W1, W2,
-                       P.coef.p);
+    launch_coef(P.n_terms, P.cspec.p, W1, W2, P.coef.p, s);
   launch_grouped_gemm(P.s1, s1A, s1B, P.X.p, compute, false, s);
   hipLaunchKernelGGL(mix_kernel, dim3(P.n_mixf_tasks), dim3(MIX_WG), 0, s, P.items.p, P.mixf_titem.p, P.mixf_tstart.p,
                      P.item_entry.p, P.ents.p, P.coef.p, P.X.p, P.Y.p);
@@ -1056,7 +1065,7 @@ static void run_fused_perblock(FusedPlan &P, const double *s1A, const double *s1
       hipLaunchKernelGGL(zero_kernel, dim3(nb256(P.out_total)), dim3(256), 0, s, P.out_total, out);
     return;
   }
-  hipLaunchKernelGGL(coef_kernel, dim3(nb256(P.n_terms)), dim3(256), 0, s, P.n_terms, P.cspec.p, W1, W2, P.coef.p);
+  launch_coef(P.n_terms, P.cspec.p, W1, W2, P.coef.p, s);
   for (int x = 0; x < P.s1.n_outs; ++x)
     launch_prepared(P.s1.a_lay, P.s1.b_lay, P.s1.signs, compute, P.pb1_cnt[x], s, P.s1.tasks.p + P.pb1_off[x], P.s1.outs.p,
                     P.s1.pairs.p, s1A, s1B, P.X.p, false);
@@ -1546,8 +1555,7 @@ int tnx_matvec_execute_stage(const tnx_plan *plan, int32_t stage, const double *
     const bool gemm_only = (stage & TNX_STAGE_GEMM_ONLY) != 0;
     switch (stage & ~TNX_STAGE_GEMM_ONLY) {
       case 0:
-        hipLaunchKernelGGL(coef_kernel, dim3(nb256(P.n_terms)), dim3(256), 0, s, P.n_terms, P.cspec.p, Wl, Wr,
-                           P.coef.p);
+        launch_coef(P.n_terms, P.cspec.p, Wl, Wr, P.coef.p, s);
         break;
       case 1:
         launch_grouped_gemm(P.s1, L, x, P.X.p, compute, false, s, !gemm_only);
