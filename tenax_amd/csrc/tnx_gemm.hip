@@ -480,20 +480,34 @@ __global__ __launch_bounds__(GEMM_THREADS) __attribute__((amdgpu_waves_per_eu(
 }
 
 // Sum the partial tiles of each split tile in slot order (deterministic) into C.
+// Each thread owns BM*BN/256 elements 256 apart (coalesced) and loads one
+// partial of all of them before adding, so its loads are in flight together.
 template <int BM, int BN>
 __global__ __launch_bounds__(256) void split_reduce_kernel(const SplitTile *__restrict__ red,
                                                            const GemmOut *__restrict__ outs,
                                                            const double *__restrict__ ws, double *__restrict__ C,
                                                            int accumulate) {
+  constexpr int PER = BM * BN / 256;
   const SplitTile r = red[blockIdx.x];
   const GemmOut o = outs[r.out];
-  for (int e = threadIdx.x; e < BM * BN; e += 256) {
+  const double *w = ws + (int64_t)r.slot0 * (BM * BN) + threadIdx.x;
+  double v[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) v[i] = 0.0;
+  for (int q = 0; q < r.nsplit; ++q) {
+    double x[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) x[i] = w[(int64_t)q * (BM * BN) + i * 256];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) v[i] += x[i];
+  }
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int e = threadIdx.x + i * 256;
     const int row = r.m0 + e / BN, col = r.n0 + e % BN;
     if (row >= o.M || col >= o.N) continue;
-    double v = 0.0;
-    for (int q = 0; q < r.nsplit; ++q) v += ws[(int64_t)(r.slot0 + q) * (BM * BN) + e];
     double *dst = C + o.c_off + (int64_t)row * o.ldc + col;
-    *dst = accumulate ? *dst + v : v;
+    *dst = accumulate ? *dst + v[i] : v[i];
   }
 }
 
