@@ -1146,28 +1146,22 @@ __device__ double block_sum(const double *part, int G) {
 }
 
 // op 0: inv = 1/(sqrt(sum)+1e-15) (initial normalisation)
-// op 1: alpha[k] = sum
-// op 2: beta[k+1] = sqrt(sum); stop if < tol, else inv = 1/beta
+// op 1: alpha[k] = sum (the re-orthogonalising step; the plain step forms it
+//       in lz_axpy_norm_kernel)
 // op 3: inv = 1/(sqrt(sum)+1e-15) (final normalisation)
+// (beta and the stop test: lz_norm_scale_kernel)
 __global__ void lz_finalize_kernel(const double *__restrict__ part, int G, int op, int k, double tol,
                                    double *__restrict__ alpha, double *__restrict__ beta, LzState *__restrict__ st) {
-  if ((op == 1 || op == 2) && st->stopped) return;
+  if (op == 1 && st->stopped) return;
   const double s = block_sum(part, G);
   if (threadIdx.x != 0) return;
   if (op == 0 || op == 3) {
     st->inv = 1.0 / (sqrt(s) + 1e-15);
   } else if (op == 1) {
     alpha[k] = s;
-  } else {
-    const double b = sqrt(s);
-    beta[k + 1] = b;
-    if (b < tol) {
-      st->stopped = 1;
-      st->nsteps = k + 1;
-    } else {
-      st->inv = 1.0 / b;
-    }
   }
+  (void)tol;
+  (void)beta;
 }
 
 __global__ void lz_multi_finalize_kernel(const double *__restrict__ part, int G, double *__restrict__ c,
@@ -1175,6 +1169,35 @@ __global__ void lz_multi_finalize_kernel(const double *__restrict__ part, int G,
   if (st->stopped) return;
   const double s = block_sum(part + (int64_t)blockIdx.x * G, G);
   if (threadIdx.x == 0) c[blockIdx.x] = s;
+}
+
+// beta[k+1] = |w| from the partials of w.w, then w /= beta[k+1], or the stop
+// (beta below tol, w left as it is).  Every workgroup forms the same sum and
+// scale factor; workgroup 0 records beta and the state.  The stop flag is
+// read and written with relaxed atomics: workgroup 0 may set it while others
+// read it, and either value they see leads them to the same (no) scaling.
+// One thread reads it for the whole workgroup, so all of its threads take the
+// same path through the barriers of block_sum.
+__global__ void lz_norm_scale_kernel(double *__restrict__ w, int64_t n, const double *__restrict__ part, int G,
+                                     int k, double tol, double *__restrict__ beta, LzState *__restrict__ st) {
+  __shared__ int stop_seen;
+  if (threadIdx.x == 0) stop_seen = __hip_atomic_load(&st->stopped, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  if (stop_seen) return;
+  const double b = sqrt(block_sum(part, G));
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    beta[k + 1] = b;
+    if (b < tol) {
+      st->nsteps = k + 1;
+      __hip_atomic_store(&st->stopped, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      st->inv = 1.0 / b;
+    }
+  }
+  if (b < tol) return;
+  const double f = 1.0 / b;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    w[i] = w[i] * f;
 }
 
 // dst = src * inv  (skipped after a stop when guarded)
@@ -1201,14 +1224,19 @@ __global__ void lz_axpy_kernel(double *__restrict__ w, const double *__restrict_
 }
 
 // The same update fused with the partial sums of w.w per workgroup (the
-// norm that follows when there is no re-orthogonalisation).
+// norm that follows when there is no re-orthogonalisation).  alpha[k] is
+// summed from the partials of <v, w> by every workgroup (the same sum in the
+// same order, so the same value everywhere; workgroup 0 records it), which
+// saves a one-workgroup launch per step.  apart and part must not overlap.
 __global__ void lz_axpy_norm_kernel(double *__restrict__ w, const double *__restrict__ v,
                                     const double *__restrict__ vp, int64_t n, int k,
-                                    const double *__restrict__ alpha, const double *__restrict__ beta,
-                                    double *__restrict__ part, const LzState *__restrict__ st) {
+                                    const double *__restrict__ apart, int G, double *__restrict__ alpha,
+                                    const double *__restrict__ beta, double *__restrict__ part,
+                                    const LzState *__restrict__ st) {
   if (st->stopped) return;
   __shared__ double sh[256];
-  const double a = alpha[k];
+  const double a = block_sum(apart, G);
+  if (blockIdx.x == 0 && threadIdx.x == 0) alpha[k] = a;
   const double b = k > 0 ? beta[k] : 0.0;
   double acc = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -1450,18 +1478,20 @@ static void lanczos_enqueue(const FusedPlan &P, const double *L, const double *W
     const double *vp = k > 0 ? V + (int64_t)(k - 1) * n : v;
     run_fused(P, L, v, Wl, Wr, R, w, compute, s, k > 0);
     hipLaunchKernelGGL(dot_partial_kernel, rg, vb, 0, s, v, w, n, part, W.st.p);
-    hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), vb, 0, s, part, LZ_G, 1, k, tol, alpha, beta, W.st.p);
+    double *npart = part;  // partials of w.w
     if (reorth) {
+      hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), vb, 0, s, part, LZ_G, 1, k, tol, alpha, beta, W.st.p);
       hipLaunchKernelGGL(lz_axpy_kernel, vg, vb, 0, s, w, v, vp, n, k, alpha, beta, W.st.p);
       hipLaunchKernelGGL(dot_multi_partial_kernel, dim3(LZ_G, k + 1), vb, 0, s, V, n, k + 1, w, part, W.st.p);
       hipLaunchKernelGGL(lz_multi_finalize_kernel, dim3(k + 1), vb, 0, s, part, LZ_G, rc, W.st.p);
       hipLaunchKernelGGL(lz_reorth_kernel, vg, vb, 0, s, w, V, n, k + 1, rc, W.st.p);
       hipLaunchKernelGGL(dot_partial_kernel, rg, vb, 0, s, w, w, n, part, W.st.p);
     } else {
-      hipLaunchKernelGGL(lz_axpy_norm_kernel, rg, vb, 0, s, w, v, vp, n, k, alpha, beta, part, W.st.p);
+      npart = part + LZ_G;  // the <v, w> partials are still being read
+      hipLaunchKernelGGL(lz_axpy_norm_kernel, rg, vb, 0, s, w, v, vp, n, k, (const double *)part, LZ_G, alpha,
+                         (const double *)beta, npart, (const LzState *)W.st.p);
     }
-    hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), vb, 0, s, part, LZ_G, 2, k, tol, alpha, beta, W.st.p);
-    hipLaunchKernelGGL(lz_scale_kernel, vg, vb, 0, s, w, w, n, W.st.p, 1);
+    hipLaunchKernelGGL(lz_norm_scale_kernel, rg, vb, 0, s, w, n, (const double *)npart, LZ_G, k, tol, beta, W.st.p);
   }
   hipLaunchKernelGGL(lz_tridiag_kernel, dim3(1), dim3(64), 0, s, alpha, beta, cvec, work, W.st.p, max_iter);
   hipLaunchKernelGGL(lz_combine_kernel, vg, vb, 0, s, theta, V, n, cvec, W.st.p);
