@@ -1099,8 +1099,18 @@ __global__ void dot_partial_kernel(const double *__restrict__ a, const double *_
   if (st && st->stopped) return;
   __shared__ double sh[256];
   double acc = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    acc += a[i] * b[i];
+  // four strided elements loaded before they are added (in the plain loop's order)
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  for (; i + 3 * step < n; i += 4 * step) {
+    const double a0 = a[i], a1 = a[i + step], a2 = a[i + 2 * step], a3 = a[i + 3 * step];
+    const double b0 = b[i], b1 = b[i + step], b2 = b[i + 2 * step], b3 = b[i + 3 * step];
+    acc += a0 * b0;
+    acc += a1 * b1;
+    acc += a2 * b2;
+    acc += a3 * b3;
+  }
+  for (; i < n; i += step) acc += a[i] * b[i];
   sh[threadIdx.x] = acc;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
@@ -1196,8 +1206,16 @@ __global__ void lz_norm_scale_kernel(double *__restrict__ w, int64_t n, const do
   }
   if (b < tol) return;
   const double f = 1.0 / b;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    w[i] = w[i] * f;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  for (; i + 3 * step < n; i += 4 * step) {
+    double x[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) x[u] = w[i + u * step];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) w[i + u * step] = x[u] * f;
+  }
+  for (; i < n; i += step) w[i] = w[i] * f;
 }
 
 // dst = src * inv  (skipped after a stop when guarded)
@@ -1239,7 +1257,24 @@ __global__ void lz_axpy_norm_kernel(double *__restrict__ w, const double *__rest
   if (blockIdx.x == 0 && threadIdx.x == 0) alpha[k] = a;
   const double b = k > 0 ? beta[k] : 0.0;
   double acc = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+  // four strided elements in flight per pass, summed in the plain loop's order
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  for (; i + 3 * step < n; i += 4 * step) {
+    double t[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) t[u] = w[i + u * step] - v[i + u * step] * a;
+    if (k > 0) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) t[u] = t[u] - vp[i + u * step] * b;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      w[i + u * step] = t[u];
+      acc += t[u] * t[u];
+    }
+  }
+  for (; i < n; i += step) {
     double t = w[i] - v[i] * a;
     if (k > 0) t = t - vp[i] * b;
     w[i] = t;
