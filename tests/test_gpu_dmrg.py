@@ -166,6 +166,29 @@ def test_device_lanczos_matches_oracle(gpu, state8, reorth):
     assert abs(abs(dot) - 1.0) < 1e-8
 
 
+@pytest.mark.parametrize("site", [0, 6])
+def test_device_lanczos_stops_on_breakdown(gpu, state8, site):
+    """Edge bonds span a Krylov space smaller than the step budget: the
+    recurrence must stop at the step where beta falls below tol (dmrg.py:1012-1014),
+    as the oracle does.  With re-orthogonalisation the breakdown beta is
+    rounding residue far below tol, so the step is fixed by the algorithm;
+    without it, beta after the space is exhausted is rounding noise (at site 6
+    the device stops at 20 steps and the oracle runs all 30), which no
+    implementation can be expected to reproduce."""
+    mpo, mps = state8
+    prob, (oL, oth, oWl, oWr, oR) = _problem(mpo, mps, site)
+    calls = []
+
+    def mv(v):
+        calls.append(1)
+        return od.matvec(oL, v, oWl, oWr, oR)
+
+    e_ref, _ = od.lanczos(mv, oth, 30, 1e-8, True)
+    e, n = prob.lanczos(30, 1e-8, True, 0)
+    assert n == len(calls) < 30
+    assert abs(e - e_ref) < 1e-10
+
+
 @pytest.mark.parametrize("reorth", [False, True])
 def test_lanczos_graph_replays_match_eager(gpu, state8, reorth):
     """The captured solve (tnx_lanczos_graph_*) reproduces the eager device
