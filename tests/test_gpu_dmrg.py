@@ -189,6 +189,25 @@ def test_device_lanczos_stops_on_breakdown(gpu, state8, site):
     assert abs(e - e_ref) < 1e-10
 
 
+@pytest.mark.parametrize("tol", [1e-8, 1e-12])
+def test_device_plain_lanczos_after_breakdown_matches_oracle(gpu, state8, tol):
+    """The reference's Lanczos (no re-orthogonalisation, dmrg.py:1004-1019) at
+    site 6, whose Krylov space is exhausted before 30 steps: the step at which
+    beta drops below tol is rounding noise (the two implementations may stop at
+    different steps), but the lowest Ritz pair is not -- energy to 1e-10 and
+    Ritz vector to 1e-8 of the oracle's plain solve."""
+    mpo, mps = state8
+    prob, (oL, oth, oWl, oWr, oR) = _problem(mpo, mps, 6)
+    e_ref, v_ref = od.lanczos(lambda v: od.matvec(oL, v, oWl, oWr, oR), oth, 30, tol, False)
+    e, n = prob.lanczos(30, tol, False, 0)
+    print(f"site 6 plain Lanczos tol={tol}: device {n} steps, dE={e - e_ref:.2e}", flush=True)
+    assert 1 <= n <= 30
+    assert abs(e - e_ref) < 1e-10
+    po = to_oracle(prob.extract(prob.present_after(n)))
+    dot = sum(float(np.sum(po.blocks[k] * v_ref.blocks[k])) for k in v_ref.blocks if k in po.blocks)
+    assert abs(abs(dot) - 1.0) < 1e-8
+
+
 @pytest.mark.parametrize("reorth", [False, True])
 def test_lanczos_graph_replays_match_eager(gpu, state8, reorth):
     """The captured solve (tnx_lanczos_graph_*) reproduces the eager device
