@@ -85,6 +85,25 @@ int tnx_lanczos(const tnx_plan *matvec_plan, const double *L, const double *Wl, 
                 const double *R, double *theta, int32_t max_iter, double tol, int32_t reorth,
                 int32_t compute, double *energy, int32_t *n_steps, void *stream);
 
+/* Allocation-free variant of tnx_lanczos.  No counterpart in the reference;
+ * it is for callers that cannot allocate device memory (code under graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): it calls no
+ * hipMalloc / hipFree and no allocating library routine.  Same arguments,
+ * kernels and results (bitwise) as tnx_lanczos.  The Krylov basis, scalars
+ * and solver state live in a static device array of
+ * TNX_LANCZOS_NOALLOC_SCRATCH_BYTES that every GPU this entry point runs on
+ * holds until the process ends; a solve needs
+ *   256 + 8 * ((max_iter + 1) * n + 1033 * (max_iter + 2))  bytes
+ * (n = the plan's theta size) and a larger one returns TNX_ERR_CAPACITY
+ * before any device work.  All calls on one GPU share that array: calls that
+ * overlap on one GPU (two host threads, or two streams) race and must be
+ * serialised by the caller.  The plan itself must have been created (and its
+ * tiling set) beforehand, since plan creation allocates. */
+#define TNX_LANCZOS_NOALLOC_SCRATCH_BYTES (64u << 20)
+int tnx_lanczos_noalloc(const tnx_plan *matvec_plan, const double *L, const double *Wl, const double *Wr,
+                        const double *R, double *theta, int32_t max_iter, double tol, int32_t reorth,
+                        int32_t compute, double *energy, int32_t *n_steps, void *stream);
+
 /* Captured Lanczos solve (hipGraph).  No counterpart in the reference, whose
  * Lanczos is a Python loop under JAX's asynchronous dispatch
  * (src/tenax/algorithms/dmrg.py:978-1045): the whole device solve of

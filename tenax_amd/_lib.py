@@ -99,6 +99,7 @@ SIGNATURES = {
                                C.POINTER(_p)]),
     "tnx_env_execute": (C.c_int, [_p, _p, _p, _p, _p, _i32, _p]),
     "tnx_lanczos": (C.c_int, [_p, _p, _p, _p, _p, _p, _i32, C.c_double, _i32, _i32, _PD, _PI32, _p]),
+    "tnx_lanczos_noalloc": (C.c_int, [_p, _p, _p, _p, _p, _p, _i32, C.c_double, _i32, _i32, _PD, _PI32, _p]),
     "tnx_lanczos_graph_create": (C.c_int, [_p, _p, _p, _p, _p, _p, _i32, C.c_double, _i32, _i32, _p,
                                            C.POINTER(_p)]),
     "tnx_lanczos_graph_update": (C.c_int, [_p, _p, _p, _p, _p, _p, _p]),

@@ -1460,6 +1460,25 @@ struct LanczosWorkspace {
   int max_iter = 0;
 };
 
+// Device memory one solve works in: the Krylov basis, the scalar arrays and
+// the solver state (from a LanczosWorkspace, or from the static scratch of
+// tnx_lanczos_noalloc).
+struct LzBufs {
+  double *V;
+  double *scal;
+  LzState *st;
+};
+
+// Doubles of the scalar arrays of a solve of max_iter steps: alpha, beta, the
+// Ritz coefficients, the re-orthogonalisation coefficients, the tridiagonal
+// solver's work (5 arrays) and LZ_G reduction partials per Krylov vector.
+static int64_t lz_nscal(int max_iter) {
+  return 3 * (int64_t)(max_iter + 2) + (max_iter + 2) + 5 * (int64_t)(max_iter + 2) +
+         (int64_t)LZ_G * (max_iter + 2);
+}
+
+static LzBufs bufs_of(LanczosWorkspace &W) { return LzBufs{W.V.p, W.scal.p, W.st.p}; }
+
 static std::mutex g_lz_mutex;
 static LanczosWorkspace g_lz;
 
@@ -1479,7 +1498,7 @@ static void lanczos_reserve(LanczosWorkspace &W, int64_t n, int max_iter) {
   TNX_REQUIRE(max_iter >= 1, TNX_ERR_ARG, "max_iter must be >= 1");
   const int64_t need_v = (int64_t)(max_iter + 1) * std::max<int64_t>(n, 1);
   if ((int64_t)W.V.n < need_v) W.V.alloc(need_v);
-  const int64_t nscal = 3 * (max_iter + 2) + (max_iter + 2) + 5 * (max_iter + 2) + (int64_t)LZ_G * (max_iter + 2);
+  const int64_t nscal = lz_nscal(max_iter);
   if ((int64_t)W.scal.n < nscal) W.scal.alloc(nscal);
   if (!W.st.p) W.st.alloc(1);
   W.n = n;
@@ -1491,55 +1510,55 @@ static void lanczos_reserve(LanczosWorkspace &W, int64_t n, int max_iter) {
 // Leaves the eigenvector in theta and energy / step count in W.st.
 static void lanczos_enqueue(const FusedPlan &P, const double *L, const double *Wl, const double *Wr,
                             const double *R, double *theta, int max_iter, double tol, int reorth, int compute,
-                            LanczosWorkspace &W, hipStream_t s) {
+                            const LzBufs &W, hipStream_t s) {
   const int64_t n = P.in_total;
-  double *alpha = W.scal.p;
+  double *alpha = W.scal;
   double *beta = alpha + (max_iter + 2);
   double *cvec = beta + (max_iter + 2);
   double *rc = cvec + (max_iter + 2);
   double *work = rc + (max_iter + 2);
   double *part = work + 5 * (max_iter + 2);
-  double *V = W.V.p;
+  double *V = W.V;
   const dim3 vg(std::min<int64_t>(std::max<int64_t>((n + 255) / 256, 1), 1024)), vb(256);
   const dim3 rg(LZ_G);
-  hipLaunchKernelGGL(lz_init_state_kernel, dim3(1), dim3(64), 0, s, W.st.p);
+  hipLaunchKernelGGL(lz_init_state_kernel, dim3(1), dim3(64), 0, s, W.st);
   // v0 = theta / (|theta| + 1e-15)
   hipLaunchKernelGGL(dot_partial_kernel, rg, vb, 0, s, theta, theta, n, part, (const LzState *)nullptr);
-  hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), vb, 0, s, part, LZ_G, 0, 0, tol, alpha, beta, W.st.p);
-  hipLaunchKernelGGL(lz_scale_kernel, vg, vb, 0, s, V, theta, n, W.st.p, 0);
+  hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), vb, 0, s, part, LZ_G, 0, 0, tol, alpha, beta, W.st);
+  hipLaunchKernelGGL(lz_scale_kernel, vg, vb, 0, s, V, theta, n, W.st, 0);
   for (int k = 0; k < max_iter; ++k) {
     double *v = V + (int64_t)k * n;
     double *w = V + (int64_t)(k + 1) * n;
     const double *vp = k > 0 ? V + (int64_t)(k - 1) * n : v;
     run_fused(P, L, v, Wl, Wr, R, w, compute, s, k > 0);
-    hipLaunchKernelGGL(dot_partial_kernel, rg, vb, 0, s, v, w, n, part, W.st.p);
+    hipLaunchKernelGGL(dot_partial_kernel, rg, vb, 0, s, v, w, n, part, W.st);
     double *npart = part;  // partials of w.w
     if (reorth) {
-      hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), vb, 0, s, part, LZ_G, 1, k, tol, alpha, beta, W.st.p);
-      hipLaunchKernelGGL(lz_axpy_kernel, vg, vb, 0, s, w, v, vp, n, k, alpha, beta, W.st.p);
-      hipLaunchKernelGGL(dot_multi_partial_kernel, dim3(LZ_G, k + 1), vb, 0, s, V, n, k + 1, w, part, W.st.p);
-      hipLaunchKernelGGL(lz_multi_finalize_kernel, dim3(k + 1), vb, 0, s, part, LZ_G, rc, W.st.p);
-      hipLaunchKernelGGL(lz_reorth_kernel, vg, vb, 0, s, w, V, n, k + 1, rc, W.st.p);
-      hipLaunchKernelGGL(dot_partial_kernel, rg, vb, 0, s, w, w, n, part, W.st.p);
+      hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), vb, 0, s, part, LZ_G, 1, k, tol, alpha, beta, W.st);
+      hipLaunchKernelGGL(lz_axpy_kernel, vg, vb, 0, s, w, v, vp, n, k, alpha, beta, W.st);
+      hipLaunchKernelGGL(dot_multi_partial_kernel, dim3(LZ_G, k + 1), vb, 0, s, V, n, k + 1, w, part, W.st);
+      hipLaunchKernelGGL(lz_multi_finalize_kernel, dim3(k + 1), vb, 0, s, part, LZ_G, rc, W.st);
+      hipLaunchKernelGGL(lz_reorth_kernel, vg, vb, 0, s, w, V, n, k + 1, rc, W.st);
+      hipLaunchKernelGGL(dot_partial_kernel, rg, vb, 0, s, w, w, n, part, W.st);
     } else {
       npart = part + LZ_G;  // the <v, w> partials are still being read
       hipLaunchKernelGGL(lz_axpy_norm_kernel, rg, vb, 0, s, w, v, vp, n, k, (const double *)part, LZ_G, alpha,
-                         (const double *)beta, npart, (const LzState *)W.st.p);
+                         (const double *)beta, npart, (const LzState *)W.st);
     }
-    hipLaunchKernelGGL(lz_norm_scale_kernel, rg, vb, 0, s, w, n, (const double *)npart, LZ_G, k, tol, beta, W.st.p);
+    hipLaunchKernelGGL(lz_norm_scale_kernel, rg, vb, 0, s, w, n, (const double *)npart, LZ_G, k, tol, beta, W.st);
   }
-  hipLaunchKernelGGL(lz_tridiag_kernel, dim3(1), dim3(64), 0, s, alpha, beta, cvec, work, W.st.p, max_iter);
-  hipLaunchKernelGGL(lz_combine_kernel, vg, vb, 0, s, theta, V, n, cvec, W.st.p);
+  hipLaunchKernelGGL(lz_tridiag_kernel, dim3(1), dim3(64), 0, s, alpha, beta, cvec, work, W.st, max_iter);
+  hipLaunchKernelGGL(lz_combine_kernel, vg, vb, 0, s, theta, V, n, cvec, W.st);
   hipLaunchKernelGGL(dot_partial_kernel, rg, vb, 0, s, theta, theta, n, part, (const LzState *)nullptr);
-  hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), vb, 0, s, part, LZ_G, 3, 0, tol, alpha, beta, W.st.p);
-  hipLaunchKernelGGL(lz_scale_kernel, vg, vb, 0, s, theta, theta, n, W.st.p, 0);
-  hipLaunchKernelGGL(lz_single_step_kernel, vg, vb, 0, s, theta, V, n, W.st.p);
+  hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), vb, 0, s, part, LZ_G, 3, 0, tol, alpha, beta, W.st);
+  hipLaunchKernelGGL(lz_scale_kernel, vg, vb, 0, s, theta, theta, n, W.st, 0);
+  hipLaunchKernelGGL(lz_single_step_kernel, vg, vb, 0, s, theta, V, n, W.st);
   TNX_CHECK_HIP(hipGetLastError());
 }
 
-static void lanczos_finish(const LanczosWorkspace &W, double *energy, int *nsteps, hipStream_t s) {
+static void lanczos_finish(const LzState *st, double *energy, int *nsteps, hipStream_t s) {
   LzState out{};
-  TNX_CHECK_HIP(hipMemcpyAsync(&out, W.st.p, sizeof(LzState), hipMemcpyDeviceToHost, s));
+  TNX_CHECK_HIP(hipMemcpyAsync(&out, st, sizeof(LzState), hipMemcpyDeviceToHost, s));
   TNX_CHECK_HIP(hipStreamSynchronize(s));
   *energy = out.energy;
   *nsteps = out.nsteps;
@@ -1550,8 +1569,64 @@ static void lanczos_run(const FusedPlan &P, const double *L, const double *Wl, c
                         int *nsteps, hipStream_t s) {
   std::lock_guard<std::mutex> lk(g_lz_mutex);
   lanczos_reserve(g_lz, P.in_total, max_iter);
-  lanczos_enqueue(P, L, Wl, Wr, R, theta, max_iter, tol, reorth, compute, g_lz, s);
-  lanczos_finish(g_lz, energy, nsteps, s);
+  lanczos_enqueue(P, L, Wl, Wr, R, theta, max_iter, tol, reorth, compute, bufs_of(g_lz), s);
+  lanczos_finish(g_lz.st.p, energy, nsteps, s);
+}
+
+// ------------------------------------------------------------------ allocation-free solve
+// The working memory of tnx_lanczos_noalloc: solver state, scalar arrays and
+// Krylov basis of one solve, a fixed array in the code object's data segment.
+// Every device the entry point runs on holds its own copy until the process
+// ends; consecutive calls on one device reuse it (the solve initialises the
+// state by a kernel and writes every basis vector and scalar before reading
+// it, so nothing left by an earlier call is read).
+static __device__ __align__(256) double scratch[TNX_LANCZOS_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+constexpr int MAX_SCRATCH_DEVICES = 64;
+static std::mutex g_scratch_mutex;
+// Device address of `scratch` per device, from hipGetSymbolAddress with that
+// device current (the host-side name `scratch` is not a device address).
+static double *g_scratch_ptr[MAX_SCRATCH_DEVICES] = {};
+
+static double *scratch_on_current_device() {
+  int dev = -1;
+  TNX_CHECK_HIP(hipGetDevice(&dev));
+  TNX_REQUIRE(dev >= 0 && dev < MAX_SCRATCH_DEVICES, TNX_ERR_ARG, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    TNX_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    TNX_REQUIRE(p != nullptr, TNX_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[dev] = (double *)p;
+  }
+  return g_scratch_ptr[dev];
+}
+
+// Layout in the scratch: LzState (one 256-byte slot), the scalar arrays, then
+// the (max_iter + 1) x n Krylov basis.  Returns the bytes a solve needs.
+static size_t noalloc_bytes(int64_t n, int max_iter) {
+  return 256 + (size_t)lz_nscal(max_iter) * sizeof(double) +
+         (size_t)(max_iter + 1) * (size_t)std::max<int64_t>(n, 1) * sizeof(double);
+}
+
+static void lanczos_run_noalloc(const FusedPlan &P, const double *L, const double *Wl, const double *Wr,
+                                const double *R, double *theta, int max_iter, double tol, int reorth, int compute,
+                                double *energy, int *nsteps, hipStream_t s) {
+  TNX_REQUIRE(max_iter >= 1, TNX_ERR_ARG, "max_iter must be >= 1");
+  TNX_REQUIRE(max_iter <= (1 << 20), TNX_ERR_CAPACITY, "max_iter exceeds the allocation-free solve's limit");
+  const size_t need = noalloc_bytes(P.in_total, max_iter);
+  if (need > (size_t)TNX_LANCZOS_NOALLOC_SCRATCH_BYTES)
+    throw Error{TNX_ERR_CAPACITY,
+                "tnx_lanczos_noalloc: the solve needs " + std::to_string(need) + " bytes of scratch ((max_iter + 1) x " +
+                    std::to_string(P.in_total) + " doubles of Krylov basis plus scalars); the static scratch holds " +
+                    std::to_string((size_t)TNX_LANCZOS_NOALLOC_SCRATCH_BYTES) + " bytes: use tnx_lanczos"};
+  double *base = scratch_on_current_device();
+  LzBufs B;
+  B.st = (LzState *)base;
+  B.scal = base + 256 / sizeof(double);
+  B.V = B.scal + lz_nscal(max_iter);
+  lanczos_enqueue(P, L, Wl, Wr, R, theta, max_iter, tol, reorth, compute, B, s);
+  lanczos_finish(B.st, energy, nsteps, s);
 }
 
 GemmProgram *fused_program(tnx_plan_s *plan, int stage) {
@@ -1706,6 +1781,20 @@ int tnx_lanczos(const tnx_plan *matvec_plan, const double *L, const double *Wl, 
   });
 }
 
+int tnx_lanczos_noalloc(const tnx_plan *matvec_plan, const double *L, const double *Wl, const double *Wr,
+                        const double *R, double *theta, int32_t max_iter, double tol, int32_t reorth,
+                        int32_t compute, double *energy, int32_t *n_steps, void *stream) {
+  return guarded([&] {
+    TNX_REQUIRE(matvec_plan && matvec_plan->kind == PlanKind::Matvec, TNX_ERR_ARG, "not a matvec plan");
+    TNX_REQUIRE(energy && n_steps, TNX_ERR_ARG, "null output");
+    TNX_REQUIRE(compute >= 0 && compute <= 2, TNX_ERR_DTYPE, "unknown compute mode");
+    const FusedPlan &P = static_cast<const FusedPlan &>(*matvec_plan);
+    int ns = 0;
+    lanczos_run_noalloc(P, L, Wl, Wr, R, theta, max_iter, tol, reorth, compute, energy, &ns, (hipStream_t)stream);
+    *n_steps = ns;
+  });
+}
+
 // ------------------------------------------------------------------ captured Lanczos solve
 }  // extern "C"
 
@@ -1738,7 +1827,7 @@ static hipGraph_t capture_lanczos(tnx_graph_s &G, const double *L, const double 
   }
   hipGraph_t g = nullptr;
   try {
-    lanczos_enqueue(*G.plan, L, Wl, Wr, R, theta, G.max_iter, G.tol, G.reorth, G.compute, G.ws, cs);
+    lanczos_enqueue(*G.plan, L, Wl, Wr, R, theta, G.max_iter, G.tol, G.reorth, G.compute, bufs_of(G.ws), cs);
   } catch (...) {
     (void)hipStreamEndCapture(cs, &g);
     if (g) (void)hipGraphDestroy(g);
@@ -1818,7 +1907,7 @@ int tnx_lanczos_graph_result(tnx_graph *graph, void *stream, double *energy, int
     TNX_REQUIRE(graph, TNX_ERR_ARG, "null graph");
     TNX_REQUIRE(energy && n_steps, TNX_ERR_ARG, "null output");
     int ns = 0;
-    lanczos_finish(graph->ws, energy, &ns, (hipStream_t)stream);
+    lanczos_finish(graph->ws.st.p, energy, &ns, (hipStream_t)stream);
     *n_steps = ns;
   });
 }
