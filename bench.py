@@ -45,34 +45,16 @@ def log(*a):
 def build_problem(L, chi, seed):
     import importlib
 
-    from tenax_amd import build_mpo_heisenberg
-    from tenax_amd.workloads import synthetic_dmrg_mps
+    import torch
+
+    from tenax_amd.workloads import centre_bond_problem
 
     pd = importlib.import_module("tenax_amd.algorithms.dmrg")
     t0 = time.time()
-    mpo_tn = build_mpo_heisenberg(L)
-    mps_tn = synthetic_dmrg_mps(L, chi, seed=seed)
-    mpo = [mpo_tn.get_tensor(i) for i in range(L)]
-    mps = [mps_tn.get_tensor(i) for i in range(L)]
-    c = L // 2 - 1
-    # the synthetic sites are not canonical: rescale each environment to
-    # max|.| = 1 so FP32 variants stay in range (timing is value-independent)
-    left = pd._build_trivial_left_env_symmetric()
-    for i in range(c):
-        left = pd._update_left_env_symmetric(left, mps[i], mpo[i])
-        left = left * (1.0 / float(left.max_abs()))
-    right = pd._build_trivial_right_env_symmetric()
-    for i in range(L - 1, c + 1, -1):
-        right = pd._update_right_env_symmetric(right, mps[i], mpo[i])
-        right = right * (1.0 / float(right.max_abs()))
-    theta, nd, pl = pd._theta_padded(mps[c], mps[c + 1])
-    theta = theta * (1.0 / float(theta.norm()))
-    prob = pd.TwoSiteProblem(theta, left, mpo[c], mpo[c + 1], right)
-    import torch
-
+    prob, ops, mps, mpo = centre_bond_problem(L, chi, seed)
     torch.cuda.synchronize()
     log(f"setup {time.time() - t0:.1f}s: theta blocks={len(prob.keys)} size={prob.total}")
-    return pd, prob, (left, mpo[c], mpo[c + 1], right, theta), mps, mpo
+    return pd, prob, ops, mps, mpo
 
 
 def committed_traffic(kernel, wgs):
@@ -144,48 +126,102 @@ def stage_times(prob, reps, compute=0):
     return out, list(f4)
 
 
-def cpu_baseline(prob, mpo_pair):
-    """Oracle per-combination matvec (reference loop structure, NumPy FP64) on the same problem."""
-    import numpy as np
+def host_info():
+    """CPU model, usable cores and the BLAS thread settings of this host."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    env = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")}
+    return {"cpu_model": model, "cpu_count": os.cpu_count(), "usable_cpus": usable, "thread_env": env}
 
+
+def _trials(fn, n):
+    """1 untimed warm-up run, then n timed runs (benchmarks/runner.py:64-117)."""
+    fn()
+    ts = []
+    for _ in range(n):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return ts
+
+
+def cpu_baseline(prob, trials=3):
+    """The oracle's per-block-combination matvec (the reference's loop structure,
+    dmrg.py:1048-1151, NumPy FP64 with OpenBLAS) on the same centre-bond problem:
+    1 warm-up + `trials` timed runs with all usable cores; the single-thread rate
+    from the same runs on a bounded sample (every 8th theta block by size) at 1
+    and at all threads, scaled by their ratio."""
+    import numpy as np
+    from threadpoolctl import threadpool_info, threadpool_limits
+
+    from oracle import core as oc
     from oracle import dmrg as od
     from tests.helpers import to_oracle
 
     L, Wl, Wr, R = prob.envs
     oL, oWl, oWr, oR = (to_oracle(t) for t in (L, Wl, Wr, R))
-    x = prob.extract(prob.present_after(1))
-    ox = to_oracle(x)
+    ox = to_oracle(prob.extract(prob.present_after(1)))
     od.MODE["eval"] = "blocks"
-    t0 = time.time()
-    y = od.matvec(oL, ox, oWl, oWr, oR)
-    dt = time.time() - t0
-    # agreement with the device result on this sample
+    info = host_info()
+    n_threads = max((p.get("num_threads") or 0) for p in threadpool_info()) or info["usable_cpus"]
+    out = {}
+    full = _trials(lambda: out.__setitem__("y", od.matvec(oL, ox, oWl, oWr, oR)), trials)
+    y = out.pop("y")
+    by_size = sorted(ox.blocks, key=lambda k: -ox.blocks[k].size)
+    ox_s = oc.SymT({k: ox.blocks[k] for k in by_size[::8]}, ox.legs, validate=False)
+    samp_all = _trials(lambda: od.matvec(oL, ox_s, oWl, oWr, oR), trials)
+    with threadpool_limits(limits=1):
+        samp_one = _trials(lambda: od.matvec(oL, ox_s, oWl, oWr, oR), trials)
+    # agreement with the device result on this problem
     yd = prob.matvec(prob.x, 0)
     from tenax_amd import SymmetricTensor
 
     yt = to_oracle(SymmetricTensor._wrap(prob.indices, prob.keys, prob.shapes, prob.offsets, yd))
     num = max(float(np.max(np.abs(yt.blocks[k] - b))) for k, b in y.blocks.items())
     den = max(float(np.max(np.abs(b))) for b in y.blocks.values())
-    return dt, num / den
+    t_full = float(np.median(full))
+    ratio = float(np.median(samp_one)) / float(np.median(samp_all))
+    return {"seconds_trials": full, "seconds": t_full, "threads": n_threads,
+            "sample_seconds_all_threads": samp_all, "sample_seconds_1_thread": samp_one,
+            "single_thread_seconds_est": t_full * ratio, "max_rel_diff_vs_gpu": num / den, "host": info}
 
 
-def run_sweep(pd, mps, mpo, chi, per_block_bonds=2):
-    """One DMRG sweep (L->R, rebuild right environments, R->L; dmrg.py:179-265) on the chi state.
+def centre_envs(pd, mps, mpo):
+    """Left and right environments of the centre bond (L/2-1, L/2) of an MPS."""
+    L = len(mpo)
+    c = L // 2 - 1
+    ops = pd._symmetric_ops()
+    left = ops.build_trivial_left_env()
+    for i in range(c):
+        left = ops.update_left_env(left, mps[i], mpo[i])
+    right = ops.build_trivial_right_env()
+    for i in range(L - 1, c + 1, -1):
+        right = ops.update_right_env(right, mps[i], mpo[i])
+    return c, left, right
 
-    Returns (sweep seconds, phase breakdown, fused and per-block seconds over the
-    same `per_block_bonds` two-site updates at the centre).
-    """
+
+def compare_updates(pd, mps, mpo, chi, reorth, n_bonds=2):
+    """Fused vs plain per-block GPU path over the same two-site updates at the
+    centre of an MPS (same inputs, no SVD): seconds and Lanczos steps."""
     import torch
 
     from tenax_amd import DMRGConfig
 
     L = len(mpo)
-    cfg = DMRGConfig(max_bond_dim=chi, num_sweeps=1, lanczos_max_iter=50)
+    cfg = DMRGConfig(max_bond_dim=chi, num_sweeps=1, lanczos_max_iter=50, lanczos_reorth=reorth)
     ops = pd._symmetric_ops()
-    mps = list(mps)
     left = pd._build_left_environments_list(mps, mpo, L, ops)
     right = pd._build_right_environments_list(mps, mpo, L, ops)
-    # fused vs per-block over the same updates (no SVD, same inputs)
     c = L // 2 - 1
     cmp = {}
     for name, fn in (("fused", ops.two_site_update), ("per_block", pd._per_block_two_site)):
@@ -193,33 +229,147 @@ def run_sweep(pd, mps, mpo, chi, per_block_bonds=2):
         steps = []
         torch.cuda.synchronize()
         t0 = time.time()
-        for i in range(c, c + per_block_bonds):
+        for i in range(c, c + n_bonds):
             fn(mps[i], mps[i + 1], left[i], mpo[i], mpo[i + 1], right[i + 2], cfg)
             steps.append(pd.LAST_STEPS["n"])
         torch.cuda.synchronize()
         cmp[name] = {"seconds": time.time() - t0, "lanczos_steps": steps}
+    cmp["bonds"] = n_bonds
+    cmp["speedup_fused"] = cmp["per_block"]["seconds"] / cmp["fused"]["seconds"]
+    return cmp
+
+
+def sweep_comparison(pd, mps_tn, mpo_tn, chi, reorth, fused_only=False):
+    """One full sweep (L->R, right environments rebuilt, R->L; dmrg.py:179-265)
+    from the same MPS on the fused/grouped path and on the plain per-block GPU
+    path (one launch per block product, Lanczos scalars through the host, one
+    SVD call per sector): seconds and phases of each."""
+    import torch
+
+    from tenax_amd import DMRGConfig, dmrg
+
+    out = {}
+    pd.PROGRESS["every"] = 10
+    for name, per_block in (("fused", False), ("per_block", True))[: 1 if fused_only else 2]:
+        cfg = DMRGConfig(max_bond_dim=chi, num_sweeps=1, lanczos_max_iter=50, lanczos_reorth=reorth,
+                         per_block=per_block)
+        pd.TIMERS.clear()
+        pd.TIMERS["enabled"] = True
+        torch.cuda.synchronize()
+        t0 = time.time()
+        res = dmrg(mpo_tn, mps_tn, cfg)
+        torch.cuda.synchronize()
+        out[name] = {"seconds": time.time() - t0, "energy": res.energy,
+                     "phases_s": {k: v for k, v in pd.TIMERS.items() if k != "enabled"}}
+        pd.TIMERS["enabled"] = False
+        log(f"sweep {name}: {out[name]['seconds']:.2f}s E={res.energy:.12f} phases {out[name]['phases_s']}")
+    pd.PROGRESS["every"] = 0
+    if not fused_only:
+        out["speedup_fused"] = out["per_block"]["seconds"] / out["fused"]["seconds"]
+        out["abs_dE_fused_vs_per_block"] = abs(out["fused"]["energy"] - out["per_block"]["energy"])
+    return out
+
+
+def segment_comparison(pd, mps, mpo, chi, reorth, n_bonds=20):
+    """Full left-to-right bond updates (two-site Lanczos, truncated SVD, left
+    environment update; dmrg.py:184-214) over the n_bonds bonds around the
+    centre, from the same MPS and environments, on the fused/grouped path and
+    on the plain per-block GPU path (one launch per block product, Lanczos
+    scalars through the host, one SVD call per sector)."""
+    import torch
+
+    from tenax_amd import DMRGConfig
+
+    L = len(mpo)
+    ops = pd._symmetric_ops()
+    left0 = pd._build_left_environments_list(mps, mpo, L, ops)
+    right = pd._build_right_environments_list(mps, mpo, L, ops)
+    c0 = L // 2 - n_bonds // 2
+    out = {"bonds": [c0, c0 + n_bonds - 1]}
+    for name, per_block in (("fused", False), ("per_block", True)):
+        cfg = DMRGConfig(max_bond_dim=chi, num_sweeps=1, lanczos_max_iter=50, lanczos_reorth=reorth,
+                         per_block=per_block)
+        update = pd._per_block_two_site if per_block else ops.two_site_update
+        sites, left = list(mps), list(left0)
+        pd._ACTIVE_CONFIG["cfg"] = cfg
+        try:
+            torch.cuda.synchronize()
+            t0 = time.time()
+            for i in range(c0, c0 + n_bonds):
+                th, e = update(sites[i], sites[i + 1], left[i], mpo[i], mpo[i + 1], right[i + 2], cfg)
+                A, s_, B, err = pd._svd_and_truncate_site(th, i, cfg)
+                sites[i], sites[i + 1] = A, B
+                left[i + 1] = ops.update_left_env(left[i], A, mpo[i])
+            torch.cuda.synchronize()
+            out[name] = {"seconds": time.time() - t0, "last_energy": float(e)}
+        finally:
+            pd._ACTIVE_CONFIG.pop("cfg", None)
+        log(f"segment {name}: {out[name]}")
+    out["speedup_fused"] = out["per_block"]["seconds"] / out["fused"]["seconds"]
+    return out
+
+
+def run_c3(pd, L=100, chi=2048, sweeps=6, seed=2, reorth=True):
+    """The L=100, chi=2048 U(1) Heisenberg DMRG from build_random_symmetric_mps(L, 4, seed)
+    (dmrg.py:125-305, 1415-1523): per-sweep energies, truncation errors, seconds and phases."""
+    import torch
+
+    from tenax_amd import DMRGConfig, build_mpo_heisenberg, build_random_symmetric_mps, dmrg
+
+    cfg = DMRGConfig(max_bond_dim=chi, num_sweeps=sweeps, lanczos_max_iter=50, lanczos_reorth=reorth)
+    mpo_tn = build_mpo_heisenberg(L)
+    mps_tn = build_random_symmetric_mps(L, 4, seed=seed)
     pd.TIMERS.clear()
     pd.TIMERS["enabled"] = True
     torch.cuda.synchronize()
     t0 = time.time()
-    for i in range(L - 1):
-        r = right[i + 2] if right[i + 2] is not None else ops.build_trivial_right_env()
-        th, e = ops.two_site_update(mps[i], mps[i + 1], left[i], mpo[i], mpo[i + 1], r, cfg)
-        A, s, B, err = pd._svd_and_truncate_site(th, i, cfg)
-        mps[i], mps[i + 1] = A, B
-        left[i + 1] = ops.update_left_env(left[i], A, mpo[i])
-    right = pd._build_right_environments_list(mps, mpo, L, ops)
-    for i in range(L - 2, -1, -1):
-        r2 = right[i + 2] if right[i + 2] is not None else ops.build_trivial_right_env()
-        th, e = ops.two_site_update(mps[i], mps[i + 1], left[i], mpo[i], mpo[i + 1], r2, cfg)
-        A, s, B, err = pd._svd_and_truncate_site(th, i, cfg, sweep_right=False)
-        mps[i], mps[i + 1] = A, B
-        right[i + 1] = ops.update_right_env(r2, B, mpo[i + 1])
+    res = dmrg(mpo_tn, mps_tn, cfg)
     torch.cuda.synchronize()
-    sweep = time.time() - t0
+    total = time.time() - t0
     phases = {k: v for k, v in pd.TIMERS.items() if k != "enabled"}
     pd.TIMERS["enabled"] = False
-    return sweep, phases, cmp
+    n_upd = 2 * (L - 1)
+    errs = res.truncation_errors
+    mps = [res.mps.get_tensor(i) for i in range(L)]
+    mpo = [mpo_tn.get_tensor(i) for i in range(L)]
+    bond_dims = [int(len(mps[i].indices[-1].charges)) for i in range(L - 1)]
+    out = {"workload": f"U(1) Heisenberg chain L={L}, Jz=Jxy=1, MPO bond 5, build_random_symmetric_mps(L, 4, "
+                       f"seed={seed}), max_bond_dim={chi}, {sweeps} sweeps, Lanczos 50 steps"
+                       f"{' with FP64 re-orthogonalisation' if reorth else ''}, FP64 MFMA matvec, FP64 SVD",
+           "seconds": total, "sweep_seconds": list(pd.LAST_RUN["sweep_seconds"]),
+           "energies_per_sweep": list(res.energies_per_sweep), "energy": res.energy,
+           "energy_per_site": res.energy / L, "converged": res.converged,
+           "max_truncation_error_per_sweep": [max(errs[k * n_upd:(k + 1) * n_upd]) for k in
+                                              range(len(res.energies_per_sweep))],
+           "max_bond_dim_reached": max(bond_dims), "phases_s": phases}
+    return out, mps, mpo, res.mps, mpo_tn
+
+
+def c3_centre_parity(pd, mps, mpo, reorth):
+    """Centre-bond two-site Lanczos of an MPS on the device vs the FP64 oracle's
+    Lanczos (stage-wise per-block products) on the same L, W, theta and R."""
+    import torch
+
+    from oracle import dmrg as od
+    from tests.helpers import to_oracle
+
+    c, left, right = centre_envs(pd, mps, mpo)
+    theta, _, _ = pd._theta_padded(mps[c], mps[c + 1])
+    prob = pd.TwoSiteProblem(theta, left, mpo[c], mpo[c + 1], right)
+    oth = to_oracle(prob.extract(prob.present_after(1)))
+    ops = [to_oracle(t) for t in (left, mpo[c], mpo[c + 1], right)]
+    torch.cuda.synchronize()
+    t0 = time.time()
+    e, n = prob.lanczos(50, 1e-12, reorth, 0)
+    torch.cuda.synchronize()
+    t_gpu = time.time() - t0
+    t0 = time.time()
+    e_ref, _ = od.lanczos(od.FactoredMatvec(*ops), oth, 50, 1e-12, reorth)
+    t_cpu = time.time() - t0
+    st = prob.plan.stats()
+    return {"bond": [c, c + 1], "theta_elems": int(prob.total), "theta_blocks": int(len(prob.keys)),
+            "flops_per_matvec": st["flops"], "energy": e, "oracle_energy": e_ref, "abs_dE": abs(e - e_ref),
+            "lanczos_steps": n, "gpu_s": t_gpu, "oracle_s": t_cpu, "reorth": reorth}, prob
 
 
 def run_c1_parity():
@@ -428,7 +578,11 @@ def main():
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--perblock-steps", type=int, default=2)
     ap.add_argument("--no-sweep", dest="sweep", action="store_false",
-                    help="skip timing one full DMRG sweep on the chi state (timed by default)")
+                    help="skip the L=100 chi=2048 DMRG run from a random MPS (run by default)")
+    ap.add_argument("--c3-sweeps", type=int, default=6)
+    ap.add_argument("--perblock-sweep", action="store_true",
+                    help="also time one full sweep from the C3 state on both paths")
+    ap.add_argument("--no-reorth", action="store_true", help="C3 with the reference's plain Lanczos")
     ap.add_argument("--workload", choices=("dmrg", "hotrg", "ctm"), default="dmrg")
     ap.add_argument("--matvec-only", action="store_true", help="fused matvec and stage timing only (profiling)")
     ap.add_argument("--calibrate-copy", action="store_true", help="run a 512 MiB device copy (counter calibration)")
@@ -574,22 +728,50 @@ def main():
         "sweep_s": None,
     }
     if args.sweep:
-        sw, phases, cmp = run_sweep(pd, mps, mpo, args.chi)
-        result["sweep_s"] = sw
-        result["sweep_phases_s"] = phases
-        result["two_site_updates_centre_s"] = {
-            "bonds": 2, "fused": cmp["fused"], "per_block": cmp["per_block"],
-            "speedup_fused": cmp["per_block"]["seconds"] / cmp["fused"]["seconds"]}
-        log(f"sweep {sw:.2f}s phases {phases} updates {cmp}")
+        c3, c3_mps, c3_mpo, c3_mps_tn, c3_mpo_tn = run_c3(pd, args.L, args.chi, args.c3_sweeps, args.seed,
+                                                          not args.no_reorth)
+        result["c3"] = c3
+        result["sweep_s"] = c3["sweep_seconds"][-1]
+        log(f"C3 {c3['seconds']:.1f}s sweeps {c3['sweep_seconds']} E {c3['energies_per_sweep']} "
+            f"phases {c3['phases_s']}")
+        par, cprob = c3_centre_parity(pd, c3_mps, c3_mpo, not args.no_reorth)
+        # the headline matvec timed on the C3 state's own centre bond as well
+        for _ in range(args.warmup):
+            cprob.matvec(cprob.x, 0)
+        cms = time_events(lambda: cprob.matvec(cprob.x, 0), args.steps)
+        par["matvec_ms"] = cms
+        par["matvec_gflops"] = par["flops_per_matvec"] / (cms * 1e-3) / 1e9
+        result["c3"]["centre_bond"] = par
+        log(f"C3 centre bond: {par}")
+        del cprob
+        cmp = compare_updates(pd, c3_mps, c3_mpo, args.chi, not args.no_reorth)
+        result["two_site_updates_centre_s"] = cmp
+        log(f"C3 centre updates fused vs per-block: {cmp}")
+        # one full sweep at chi = 2048 everywhere (the C3 run's last left-to-right
+        # half still had chi = 1024 bonds), and the same sweep on the per-block
+        # path (about 3 minutes; a 20-bond centre segment by default)
+        if args.perblock_sweep:
+            sc = sweep_comparison(pd, c3_mps_tn, c3_mpo_tn, args.chi, not args.no_reorth)
+        else:
+            sc = sweep_comparison(pd, c3_mps_tn, c3_mpo_tn, args.chi, not args.no_reorth, fused_only=True)
+            sc["segment"] = segment_comparison(pd, c3_mps, c3_mpo, args.chi, not args.no_reorth)
+        result["sweep_vs_per_block"] = sc
+        result["sweep_s"] = sc["fused"]["seconds"]
     if not args.no_cpu:
-        dt, err = cpu_baseline(prob, (Wl, Wr))
-        cores = os.cpu_count()
-        result["cpu_baseline"] = {"value": flops / dt / 1e9, "unit": "GFLOP/s", "cores": cores, "kind": "port",
-                                  "sample": "1 matvec of the same centre-bond problem by the oracle's "
-                                            "per-block-combination loop (NumPy FP64, OpenBLAS threads = cores); "
-                                            "value = planner flops / CPU time",
-                                  "seconds": dt, "max_rel_diff_vs_gpu": err}
-        log(f"cpu baseline {dt:.2f}s rel diff {err:.2e}")
+        cb = cpu_baseline(prob)
+        result["cpu_baseline"] = {
+            "value": flops / cb["seconds"] / 1e9, "unit": "GFLOP/s", "cores": cb["threads"], "kind": "port",
+            "sample": "1 matvec of the headline centre-bond problem by the oracle's per-block-combination loop "
+                      "(the reference's loop structure; NumPy FP64, OpenBLAS), 1 warm-up + 3 trials, median; "
+                      "value = planner flops / CPU time",
+            "single_thread_value": flops / cb["single_thread_seconds_est"] / 1e9,
+            "single_thread_sample": "every 8th theta block by size (1 warm-up + 3 trials at 1 and at all "
+                                    "threads); full-matvec time scaled by the sample's 1-thread/all-thread ratio",
+            **{k: cb[k] for k in ("seconds", "seconds_trials", "sample_seconds_all_threads",
+                                  "sample_seconds_1_thread", "single_thread_seconds_est", "max_rel_diff_vs_gpu",
+                                  "host")}}
+        log(f"cpu baseline {cb['seconds']:.2f}s ({cb['threads']} threads), 1 thread est "
+            f"{cb['single_thread_seconds_est']:.1f}s, rel diff {cb['max_rel_diff_vs_gpu']:.2e}")
     if not args.no_parity:
         result["parity"] = run_c1_parity()
         log(f"parity {result['parity']}")

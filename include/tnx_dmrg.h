@@ -73,6 +73,11 @@ int tnx_env_plan(int32_t side, const tnx_blocks *E, const tnx_blocks *A, const t
                  const tnx_leg *out_legs, void *stream, tnx_plan **plan);
 int tnx_env_execute(const tnx_plan *plan, const double *E, const double *A, const double *W,
                     double *out, int32_t compute, void *stream);
+/* Plain per-block GPU path of the same update: one GEMM launch per block
+ * product of each stage (the per-combination structure of dmrg.py:1165-1232
+ * on a GPU); the comparison baseline of the sweep. */
+int tnx_env_execute_perblock(const tnx_plan *plan, const double *E, const double *A, const double *W,
+                             double *out, int32_t compute, void *stream);
 
 /* Lanczos ground state of the planned matvec (dmrg.py:978-1045), entirely on
  * the device: FP64 alpha/beta and vector updates, optional one-pass classical

@@ -98,6 +98,7 @@ SIGNATURES = {
     "tnx_env_plan": (C.c_int, [_i32, C.POINTER(Blocks), C.POINTER(Blocks), C.POINTER(Blocks), C.POINTER(Leg), _p,
                                C.POINTER(_p)]),
     "tnx_env_execute": (C.c_int, [_p, _p, _p, _p, _p, _i32, _p]),
+    "tnx_env_execute_perblock": (C.c_int, [_p, _p, _p, _p, _p, _i32, _p]),
     "tnx_lanczos": (C.c_int, [_p, _p, _p, _p, _p, _p, _i32, C.c_double, _i32, _i32, _PD, _PI32, _p]),
     "tnx_lanczos_noalloc": (C.c_int, [_p, _p, _p, _p, _p, _p, _i32, C.c_double, _i32, _i32, _PD, _PI32, _p]),
     "tnx_lanczos_graph_create": (C.c_int, [_p, _p, _p, _p, _p, _p, _i32, C.c_double, _i32, _i32, _p,

@@ -50,7 +50,10 @@ class DMRGConfig:
     Extra fields: ``lanczos_reorth`` (one classical Gram-Schmidt pass per
     Lanczos step, FP64; off like the reference), ``compute`` ("f64" FP64 MFMA,
     "f32", "split_f32"; None = the global contraction setting) and
-    ``per_block`` (run the matvec as the plain per-block GPU path).
+    ``per_block`` (run the sweep as the plain per-block GPU path, the
+    reference's loop structure on the device: one launch per block product in
+    matvecs and environment updates, Lanczos scalars through the host, one SVD
+    call per sector; the comparison baseline).
     """
 
     max_bond_dim: int = 100
@@ -180,7 +183,8 @@ def _env_update(side: int, env: SymmetricTensor, site: SymmetricTensor, mpo: Sym
     plan, keys, shapes, offs, tot = plans.env_plan(0, env, A, W, out_idx)
     out = torch.empty(max(tot, 0), dtype=torch.float64, device=env.data.device)
     lib = _lib.load()
-    _lib.check(lib.tnx_env_execute(plan.ptr, _lib.dptr(env.data), _lib.dptr(A.data), _lib.dptr(W.data),
+    fn = lib.tnx_env_execute_perblock if (config is not None and config.per_block) else lib.tnx_env_execute
+    _lib.check(fn(plan.ptr, _lib.dptr(env.data), _lib.dptr(A.data), _lib.dptr(W.data),
                                    _lib.dptr(out), _compute(config), _lib.stream_handle()))
     return SymmetricTensor._wrap(out_idx, keys, shapes, offs, out)
 
@@ -206,6 +210,19 @@ _ACTIVE_CONFIG: dict = {}
 TIMERS: dict = {"enabled": False}
 # Krylov steps taken by the most recent two-site update (either path).
 LAST_STEPS: dict = {"n": 0}
+# Wall-clock seconds of each sweep of the most recent dmrg() call (device
+# synchronised at sweep ends only when the phase timers are enabled).
+LAST_RUN: dict = {"sweep_seconds": []}
+# Progress lines on stderr every `every` two-site updates (0 = silent).
+PROGRESS: dict = {"every": 0}
+
+
+def _progress(sweep, i, direction, energy):
+    n = PROGRESS["every"]
+    if n and i % n == 0:
+        import sys
+
+        print(f"[dmrg] sweep {sweep + 1} {direction} bond {i} E={energy:.12f}", file=sys.stderr, flush=True)
 
 
 class _phase:
@@ -454,7 +471,8 @@ def _svd_and_truncate_site(theta, site: int, config: DMRGConfig, sweep_right: bo
     bond = f"v{site}_{site + 1}"
     with _phase("svd"):
         A, s, B, s_full = truncated_svd_symmetric(theta, left, right, bond, config.max_bond_dim,
-                                                  config.svd_trunc_err, absorb="right" if sweep_right else "left")
+                                                  config.svd_trunc_err, absorb="right" if sweep_right else "left",
+                                                  per_sector=config.per_block)
     n_keep = len(s)
     if len(s_full) > n_keep:
         tot = float(torch.sum(s_full ** 2))
@@ -506,7 +524,11 @@ def dmrg(hamiltonian: TensorNetwork, initial_mps: TensorNetwork, config: DMRGCon
         energies, errs = [], []
         energy, converged = 0.0, False
         update = _per_block_two_site if config.per_block else ops.two_site_update
+        import time as _time
+
+        LAST_RUN["sweep_seconds"] = []
         for sweep in range(config.num_sweeps):
+            t_sweep = _time.perf_counter()
             prev = energy
             if sweep > 0:
                 left = _build_left_environments_list(mps, mpo, L, ops)
@@ -518,6 +540,7 @@ def dmrg(hamiltonian: TensorNetwork, initial_mps: TensorNetwork, config: DMRGCon
                 mps[i], mps[i + 1] = A, B
                 errs.append(err)
                 left[i + 1] = ops.update_left_env(left[i], A, mpo[i])
+                _progress(sweep, i, "L->R", energy)
             right = _build_right_environments_list(mps, mpo, L, ops)
             for i in range(L - 2, -1, -1):
                 r2 = right[i + 2] if right[i + 2] is not None else ops.build_trivial_right_env()
@@ -527,7 +550,9 @@ def dmrg(hamiltonian: TensorNetwork, initial_mps: TensorNetwork, config: DMRGCon
                 mps[i], mps[i + 1] = A, B
                 errs.append(err)
                 right[i + 1] = ops.update_right_env(r2, B, mpo[i + 1])
+                _progress(sweep, i, "R->L", energy)
             energies.append(energy)
+            LAST_RUN["sweep_seconds"].append(_time.perf_counter() - t_sweep)
             if config.verbose:
                 print(f"Sweep {sweep + 1}/{config.num_sweeps}: E = {energy:.10f}")
             if config.target_charge is not None:
@@ -562,24 +587,29 @@ def _per_block_two_site(site_l, site_r, left_env, mpo_l, mpo_r, right_env, confi
     prob = TwoSiteProblem(theta, left_env, mpo_l, mpo_r, right_env)
     comp = _compute(config)
     x = prob.x
-    v = x * (1.0 / (float(torch.linalg.vector_norm(x)) + 1e-15))
-    basis, alphas, betas = [v], [], [0.0]
-    for step in range(config.lanczos_max_iter):
-        w = prob.matvec(basis[-1], comp, per_block=True)
-        a = float(torch.dot(basis[-1], w))
+    m = config.lanczos_max_iter
+    V = torch.empty((m + 1, x.numel()), dtype=torch.float64, device=x.device)
+    V[0] = x * (1.0 / (float(torch.linalg.vector_norm(x)) + 1e-15))
+    alphas, betas = [], [0.0]
+    nv = 1
+    for step in range(m):
+        v = V[nv - 1]
+        w = prob.matvec(v, comp, per_block=True)
+        a = float(torch.dot(v, w))  # host round trip per scalar, as dmrg.py:1005-1012
         alphas.append(a)
-        w = w - basis[-1] * a
+        w = w - v * a
         if step > 0:
-            w = w - basis[-2] * betas[-1]
+            w = w - V[nv - 2] * betas[-1]
         if config.lanczos_reorth:
-            cs = [float(torch.dot(b, w)) for b in basis]
-            for b, c in zip(basis, cs):
-                w = w - b * c
+            c = V[:nv] @ w
+            w = w - V[:nv].T @ c
         b = float(torch.linalg.vector_norm(w))
         betas.append(b)
         if b < config.lanczos_tol:
             break
-        basis.append(w * (1.0 / b))
+        V[nv] = w * (1.0 / b)
+        nv += 1
+    basis = [V[i] for i in range(nv)]
     n = len(alphas)
     LAST_STEPS["n"] = n
     if n == 1:

@@ -1768,6 +1768,16 @@ int tnx_env_execute(const tnx_plan *plan, const double *E, const double *A, cons
   });
 }
 
+int tnx_env_execute_perblock(const tnx_plan *plan, const double *E, const double *A, const double *W, double *out,
+                             int32_t compute, void *stream) {
+  return guarded([&] {
+    TNX_REQUIRE(plan && (plan->kind == PlanKind::EnvLeft || plan->kind == PlanKind::EnvRight), TNX_ERR_ARG,
+                "not an environment plan");
+    FusedPlan &P = const_cast<FusedPlan &>(static_cast<const FusedPlan &>(*plan));
+    run_fused_perblock(P, E, A, W, nullptr, A, out, compute, (hipStream_t)stream);
+  });
+}
+
 int tnx_lanczos(const tnx_plan *matvec_plan, const double *L, const double *Wl, const double *Wr, const double *R,
                 double *theta, int32_t max_iter, double tol, int32_t reorth, int32_t compute, double *energy,
                 int32_t *n_steps, void *stream) {

@@ -20,21 +20,26 @@ order, exactly as the reference's linalg.py:181 does (see _assemble).
 
 Decomposition.  Without ``absorb`` each sector matrix goes through
 torch.linalg.svd (rocSOLVER gesvd) in float64, and so does the DMRG split
-(``absorb``) while every sector is at most SMALL_SECTOR wide.  Above that
-size the split uses the fact that only one factor has to be an isometry: the
-other one carries the singular values.  The isometric side then comes from a
-Gram eigenproblem,
+(``absorb``) while every sector is at most SMALL_SECTOR wide (sectors of one
+shape in one batched call).  Above that size the split uses the fact that
+only one factor has to be an isometry: the other one carries the singular
+values.  The isometric side comes from a Gram eigenproblem,
 ``M M^T = U diag(s^2) U^T`` (absorb="right") or ``M^T M = V diag(s^2) V^T``
-(absorb="left"), and the carried side is the exact product ``U^T M`` or
-``M V``; s is the row (column) norm of that product.  Eigenvectors are
-orthonormal to working precision whatever the spectrum, the carried factor
-needs no division by s, and ``U U^T M`` is the optimal rank-k approximation up
-to eigenvector error eps*s_max^2/gap, which only matters for the truncation
-when two singular values straddling the cut are degenerate to that level.
-The eigenproblems of all sectors are solved in one batched torch.linalg.eigh
-call: Gram matrices are scaled by their trace and padded to a common size
+(absorb="left").  (Solving the smaller side's Gram matrix instead and
+re-orthonormalising M V was tried: its columns for singular values below
+~1e-8 s_max are not determined by the product, and the Householder QR that
+then has to complete them costs as much as the larger eigenproblem.)  The carried side
+is the exact product ``U^T M`` or ``M V``; s is the row (column) norm of that
+product.  The carried factor needs no division by s, and ``U U^T M`` is the
+optimal rank-k approximation up to eigenvector error eps*s_max^2/gap, which
+only matters for the truncation when two singular values straddling the cut
+are degenerate to that level.  The eigenproblems of all sectors are solved
+in batched torch.linalg.eigh calls (size groups run concurrently on side
+streams): Gram matrices are scaled by their trace and padded to a common size
 with -1 on the padded diagonal, so padded eigenpairs sit at -1 and the
-sector's own ones in [0, 1] come out exactly decoupled.
+sector's own ones in [0, 1] come out exactly decoupled.  ``per_sector=True``
+instead runs one rocSOLVER SVD per sector in sector order (the reference's
+loop, linalg.py:163-190): the per-block comparison path.
 """
 
 from __future__ import annotations
@@ -200,7 +205,23 @@ def _eigvecs_desc(grams, pad_value=-1.0, normalise=True):
 
 # largest min(rows, cols) of any sector for which the DMRG split uses a
 # direct SVD per sector instead of the batched Gram eigenproblems
-SMALL_SECTOR = 256
+SMALL_SECTOR = 64
+
+
+def _factor_per_sector(mats, absorb):
+    """One rocSOLVER SVD call per sector matrix, in sector order (the loop of
+    reference linalg.py:163-190 on the device; the per-block comparison path),
+    with s absorbed by a product when asked."""
+    out = []
+    for M in mats:
+        U, sv, Vh = torch.linalg.svd(M, full_matrices=False)
+        if absorb == "right":
+            out.append((U, sv, Vh * sv[:, None]))
+        elif absorb == "left":
+            out.append((U * sv[None, :], sv, Vh))
+        else:
+            out.append((U, sv, Vh))
+    return out
 
 
 def _factor_all(mats, absorb):
@@ -222,11 +243,17 @@ def _factor_all(mats, absorb):
     if absorb in ("left", "right") and max(min(M.shape) for M in mats) <= SMALL_SECTOR:
         # small problems: a direct SVD per sector (rocSOLVER gesvd, cheap at
         # this size) rounds singular values like LAPACK, which decides
-        # near-ties at the truncation cut the same way as the reference
-        out = []
-        for M in mats:
-            U, sv, Vh = torch.linalg.svd(M, full_matrices=False)
-            out.append((U, sv, Vh * sv[:, None]) if absorb == "right" else (U * sv[None, :], sv, Vh))
+        # near-ties at the truncation cut the same way as the reference;
+        # sectors of one shape (the +-q pairs) share one batched call
+        out = [None] * len(mats)
+        by_shape: dict = {}
+        for j, M in enumerate(mats):
+            by_shape.setdefault(tuple(M.shape), []).append(j)
+        for js in by_shape.values():
+            U, sv, Vh = torch.linalg.svd(torch.stack([mats[j] for j in js]), full_matrices=False)
+            for b, j in enumerate(js):
+                out[j] = ((U[b], sv[b], Vh[b] * sv[b][:, None]) if absorb == "right"
+                          else (U[b] * sv[b][None, :], sv[b], Vh[b]))
         return out
     if absorb == "isometries":
         augs = []
@@ -337,7 +364,7 @@ def _assemble(t: SymmetricTensor, lax, rax):
 
 
 def truncated_svd_symmetric_many(tensors, left_labels, right_labels, new_bond_label, max_singular_values=None,
-                                 max_truncation_err=None, normalize=False, absorb=None):
+                                 max_truncation_err=None, normalize=False, absorb=None, per_sector=False):
     """truncated_svd_symmetric over several tensors with the same labels, whose
     sector eigenproblems share one batched eigh (e.g. many HOTRG instances).
     Each tensor is truncated on its own (global order within the tensor)."""
@@ -350,8 +377,11 @@ def truncated_svd_symmetric_many(tensors, left_labels, right_labels, new_bond_la
         sectors, q_list, mats = _assemble(t, lax, rax)
         prepared.append((t, lax, rax, sectors, q_list, len(all_mats)))
         all_mats.extend(mats)
-    factors = _factor_all(all_mats, absorb)
-    if absorb in ("left", "right") and REFINE > 0:
+    if per_sector:
+        factors = _factor_per_sector(all_mats, absorb)
+    else:
+        factors = _factor_all(all_mats, absorb)
+    if absorb in ("left", "right") and REFINE > 0 and not per_sector:
         _refine_small(prepared, all_mats, factors, absorb, max_singular_values, max_truncation_err)
     return [_truncate_and_pack(t, lax, rax, sectors, q_list, all_mats[j0:j0 + len(q_list)],
                                factors[j0:j0 + len(q_list)], new_bond_label, max_singular_values,
@@ -404,7 +434,8 @@ def _refine_small(prepared, mats, factors, absorb, max_singular_values, max_trun
 
 
 def truncated_svd_symmetric(tensor: SymmetricTensor, left_labels, right_labels, new_bond_label,
-                            max_singular_values=None, max_truncation_err=None, normalize=False, absorb=None):
+                            max_singular_values=None, max_truncation_err=None, normalize=False, absorb=None,
+                            per_sector=False):
     """Returns (U, s, Vh, s_full).
 
     ``absorb``: None (plain SVD), "right"/"left" (s carried by Vh / U, the DMRG
@@ -412,7 +443,7 @@ def truncated_svd_symmetric(tensor: SymmetricTensor, left_labels, right_labels, 
     isometries as from the SVD, s not absorbed; the HOTRG use, hotrg.py:137-146).
     """
     return truncated_svd_symmetric_many([tensor], left_labels, right_labels, new_bond_label, max_singular_values,
-                                        max_truncation_err, normalize, absorb)[0]
+                                        max_truncation_err, normalize, absorb, per_sector)[0]
 
 
 def _truncate_and_pack(t, lax, rax, sectors, q_list, mats, factors, new_bond_label, max_singular_values,

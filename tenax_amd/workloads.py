@@ -120,3 +120,34 @@ def synthetic_dmrg_mps(L: int, chi: int, seed: int = 2, width2: float = 18.0):
     for i in range(L - 1):
         mps.connect(i, f"v{i}_{i + 1}", i + 1, f"v{i}_{i + 1}")
     return mps
+
+
+def centre_bond_problem(L: int, chi: int, seed: int = 2):
+    """The two-site problem at the centre bond (L/2 - 1, L/2) of the Heisenberg
+    chain with the converged-state charge profile (synthetic_dmrg_mps): left and
+    right environments built on the device from the synthetic sites, each
+    rescaled to max|.| = 1 per site (the synthetic sites are not canonical; the
+    scale keeps FP32 variants in range and changes no structure), theta
+    normalised.  Returns (problem, (left, Wl, Wr, right, theta), mps, mpo)."""
+    import importlib
+
+    from tenax_amd.algorithms.dmrg import build_mpo_heisenberg
+
+    pd = importlib.import_module("tenax_amd.algorithms.dmrg")
+    mpo_tn = build_mpo_heisenberg(L)
+    mps_tn = synthetic_dmrg_mps(L, chi, seed=seed)
+    mpo = [mpo_tn.get_tensor(i) for i in range(L)]
+    mps = [mps_tn.get_tensor(i) for i in range(L)]
+    c = L // 2 - 1
+    left = pd._build_trivial_left_env_symmetric()
+    for i in range(c):
+        left = pd._update_left_env_symmetric(left, mps[i], mpo[i])
+        left = left * (1.0 / float(left.max_abs()))
+    right = pd._build_trivial_right_env_symmetric()
+    for i in range(L - 1, c + 1, -1):
+        right = pd._update_right_env_symmetric(right, mps[i], mpo[i])
+        right = right * (1.0 / float(right.max_abs()))
+    theta, _, _ = pd._theta_padded(mps[c], mps[c + 1])
+    theta = theta * (1.0 / float(theta.norm()))
+    prob = pd.TwoSiteProblem(theta, left, mpo[c], mpo[c + 1], right)
+    return prob, (left, mpo[c], mpo[c + 1], right, theta), mps, mpo

@@ -159,3 +159,54 @@ def test_vector_algebra_union_keys_matches_oracle(gpu):
     # inner of a tensor with itself is its squared norm
     assert abs(float(inner(B, B)) - oB.norm() ** 2) <= 1e-13 * oB.norm() ** 2
     torch.cuda.synchronize()
+
+
+@pytest.fixture(scope="module")
+def c3_centre():
+    """The chi=2048 centre-bond problem of the L=100 chain (converged-state charge profile, seed 2)."""
+    from tenax_amd.workloads import centre_bond_problem
+
+    prob, (L, Wl, Wr, R, theta), _, _ = centre_bond_problem(100, 2048, 2)
+    ops = tuple(to_oracle(t) for t in (L, Wl, Wr, R))
+    oth = to_oracle(prob.extract(prob.present_after(1)))
+    return prob, ops, oth
+
+
+def test_c3_centre_matvec_matches_oracle(gpu, c3_centre):
+    """One fused FP64 matvec at chi=2048 against the oracle's factored per-block
+    products (same sum, same reached block set)."""
+    prob, (oL, oWl, oWr, oR), oth = c3_centre
+    assert int(prob.total) == 2986058 and len(prob.keys) == 46
+    ref = od.FactoredMatvec(oL, oWl, oWr, oR)(oth)
+    y = prob.matvec(prob.x, 0)
+    from tenax_amd import SymmetricTensor
+
+    got = to_oracle(SymmetricTensor._wrap(prob.indices, prob.keys, prob.shapes, prob.offsets, y))
+    scale = max(float(np.abs(b).max()) for b in ref.blocks.values())
+    err = max(float(np.abs(got.blocks[k] - b).max()) for k, b in ref.blocks.items()) / scale
+    print(f"C3 centre matvec max rel err {err:.2e} ({len(ref.blocks)} blocks)", flush=True)
+    assert err < 1e-13
+    for k, b in got.blocks.items():
+        if k not in ref.blocks:
+            assert not np.any(b)
+
+
+@pytest.mark.parametrize("reorth", [False, True])
+def test_c3_centre_lanczos_matches_oracle(gpu, c3_centre, reorth):
+    """The 50-step device Lanczos at the chi=2048 centre bond (dmrg.py:978-1045,
+    FP64 MFMA matvecs) against the oracle's FP64 Lanczos on the same L, W, theta
+    and R: |dE| < 1e-10 and the same Ritz vector."""
+    prob, (oL, oWl, oWr, oR), oth = c3_centre
+    x0 = prob.x.clone()
+    try:
+        e, n = prob.lanczos(50, 1e-12, reorth, 0)
+        v = to_oracle(prob.extract(prob.present_after(n)))
+    finally:
+        prob.x.copy_(x0)
+    e_ref, v_ref = od.lanczos(od.FactoredMatvec(oL, oWl, oWr, oR), oth, 50, 1e-12, reorth)
+    print(f"C3 centre Lanczos reorth={reorth}: E={e:.13f} oracle {e_ref:.13f} dE={e - e_ref:.2e} steps={n}",
+          flush=True)
+    assert n == 50
+    assert abs(e - e_ref) < 1e-10
+    dot = sum(float(np.sum(v.blocks[k] * b)) for k, b in v_ref.blocks.items() if k in v.blocks)
+    assert abs(abs(dot) - 1.0) < 1e-8
