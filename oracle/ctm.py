@@ -132,8 +132,27 @@ def canonical_top(w, V, k):
     return canonical_signs(out)
 
 
+# Projector freezing: "record" appends every projector computed, "replay"
+# returns the recorded ones in order (i counts).  The reference computes its
+# projectors under stop_gradient (_ctm_tensor.py:233-296), so the derivative
+# its implicit backward takes is that of the CTM map with the projectors held
+# fixed; finite differences of the frozen map check it.
+FROZEN = {"mode": None, "list": [], "i": 0}
+
+
 def projector(C1g, C4g, chi, method):
     """_ctm_tensor.py:233-296 (dense path); P: (fused, k), canonical basis."""
+    if FROZEN["mode"] == "replay":
+        P = FROZEN["list"][FROZEN["i"]]
+        FROZEN["i"] += 1
+        return P
+    P = _projector(C1g, C4g, chi, method)
+    if FROZEN["mode"] == "record":
+        FROZEN["list"].append(P)
+    return P
+
+
+def _projector(C1g, C4g, chi, method):
     if method not in ("eigh", "qr"):
         raise ValueError(f"Unknown projector_method={method!r}; expected 'eigh' or 'qr'.")
     if method == "qr":
@@ -338,3 +357,102 @@ def loss(A, chi, H, max_iter=100, conv_tol=1e-10, method="qr"):
     An = A / (np.linalg.norm(A) + 1e-10)
     env, _ = ctm_fixed_point(An, chi, max_iter, conv_tol, True, method)
     return energy(An, env, H)
+
+
+def frozen_loss(A, env_star, projs, chi, H, n_iter, method="qr"):
+    """loss() with the projectors held at `projs` (recorded by one sweep at the
+    fixed point env_star of the unperturbed tensor): n_iter gauge-fixed
+    sweeps of the frozen map from env_star, then the energy."""
+    An = A / (np.linalg.norm(A) + 1e-10)
+    a = double_layer(An)
+    env = env_star
+    try:
+        FROZEN.update(mode="replay", list=projs, i=0)
+        for _ in range(n_iter):
+            FROZEN["i"] = 0
+            env = gauge_fix(sweep(env, a, chi, True, method))
+    finally:
+        FROZEN.update(mode=None, list=[], i=0)
+    return energy(An, env, H)
+
+
+def frozen_fixed_point(A, chi, n_iter, method="qr"):
+    """Fixed point of the normalised tensor (n_iter gauge-fixed sweeps) and the
+    four projectors of one more sweep at it."""
+    An = A / (np.linalg.norm(A) + 1e-10)
+    env, _ = ctm_fixed_point(An, chi, n_iter, 0.0, True, method)
+    try:
+        FROZEN.update(mode="record", list=[], i=0)
+        sweep(env, double_layer(An), chi, True, method)
+        projs = list(FROZEN["list"])
+    finally:
+        FROZEN.update(mode=None, list=[], i=0)
+    return env, projs
+
+
+def frozen_gradient_fd(A, chi, H, n_iter, h=1e-5, method="qr", directions=None):
+    """Central differences (step h) of frozen_loss around A: the full gradient,
+    or the directional derivatives along `directions`."""
+    env, projs = frozen_fixed_point(A, chi, n_iter, method)
+
+    def d(u):
+        return (frozen_loss(A + h * u, env, projs, chi, H, n_iter, method)
+                - frozen_loss(A - h * u, env, projs, chi, H, n_iter, method)) / (2 * h)
+
+    if directions is not None:
+        return [d(u) for u in directions]
+    g = np.zeros_like(A)
+    gf = g.reshape(-1)
+    for i in range(gf.size):
+        e = np.zeros(A.size)
+        e[i] = 1.0
+        gf[i] = d(e.reshape(A.shape))
+    return g
+
+
+def loss_gradient_fd(A, chi, H, h=1e-5, max_iter=100, conv_tol=1e-12, method="qr"):
+    """Central finite differences of loss() in every entry of A (2 * A.size
+    converged forward runs): the gradient the reference takes by implicit
+    differentiation of ctm_tensor_converge (ad_utils.py:694-764)."""
+    g = np.zeros_like(A)
+    flat = A.reshape(-1)
+    gf = g.reshape(-1)
+    for i in range(flat.size):
+        e = np.zeros_like(flat)
+        e[i] = h
+        lp = loss((flat + e).reshape(A.shape), chi, H, max_iter, conv_tol, method)
+        lm = loss((flat - e).reshape(A.shape), chi, H, max_iter, conv_tol, method)
+        gf[i] = (lp - lm) / (2 * h)
+    return g
+
+
+def optimize_gs(A, chi, H, lr, num_steps, gs_conv_tol=1e-8, max_iter=100, conv_tol=1e-12, method="qr",
+                grad=None):
+    """ipeps.py:2370-2434 (1-site): Adam with optax's defaults (b1=0.9,
+    b2=0.999, eps=1e-8, bias-corrected moments) on A, renormalised after each
+    update; the best-energy A is returned with its converged energy.
+    ``grad(A)`` supplies the gradient of loss() (central differences by
+    default)."""
+    grad = grad or (lambda x: loss_gradient_fd(x, chi, H, max_iter=max_iter, conv_tol=conv_tol, method=method))
+    A = A / (np.linalg.norm(A) + 1e-10)
+    m = np.zeros_like(A)
+    v = np.zeros_like(A)
+    best_e, best_A, prev = float("inf"), A, float("inf")
+    energies = []
+    for step in range(num_steps):
+        e = loss(A, chi, H, max_iter, conv_tol, method)
+        energies.append(e)
+        g = grad(A)
+        if e < best_e:
+            best_e, best_A = e, A
+        if abs(e - prev) < gs_conv_tol:
+            break
+        prev = e
+        m = 0.9 * m + 0.1 * g
+        v = 0.999 * v + 0.001 * g * g
+        mh = m / (1 - 0.9 ** (step + 1))
+        vh = v / (1 - 0.999 ** (step + 1))
+        A = A - lr * mh / (np.sqrt(vh) + 1e-8)
+        A = A / (np.linalg.norm(A) + 1e-10)
+    Af = best_A / (np.linalg.norm(best_A) + 1e-10)
+    return Af, loss(Af, chi, H, max_iter, conv_tol, method), energies

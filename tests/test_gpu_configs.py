@@ -210,3 +210,228 @@ def test_c3_centre_lanczos_matches_oracle(gpu, c3_centre, reorth):
     assert abs(e - e_ref) < 1e-10
     dot = sum(float(np.sum(v.blocks[k] * b)) for k, b in v_ref.blocks.items() if k in v.blocks)
     assert abs(abs(dot) - 1.0) < 1e-8
+
+
+# ------------------------------------------------------------------ iPEPS CTMRG (D=4)
+def _ctm():
+    import importlib
+
+    return importlib.import_module("tenax_amd.algorithms.ctm")
+
+
+def _c4_tensor():
+    """The D=4, d=2 site tensor N(0, 1) from PRNGKey(3) (the build's generator), normalised."""
+    from tenax_amd.core.tensor import PRNGKey, normal
+
+    A = normal(PRNGKey(3), (4, 4, 4, 4, 2))
+    return A / np.linalg.norm(A)
+
+
+def _near_product(D, eps, seed=11):
+    """A D x D x D x D x 2 tensor close to the product state (0.8, -0.35): the
+    CTM fixed point then exists and is reached (the generic seed-3 tensor's
+    reference CTM map does not converge)."""
+    rng = np.random.default_rng(seed)
+    A = eps * rng.standard_normal((D, D, D, D, 2))
+    A[0, 0, 0, 0, :] += np.array([0.8, -0.35])
+    return A / np.linalg.norm(A)
+
+
+@pytest.mark.parametrize("symmetric", [False, True])
+def test_c4_sweep_at_size_matches_oracle(gpu, symmetric):
+    """One CTM sweep (left, right, top, bottom moves with QR projectors,
+    _ctm_tensor.py:586-810) at D=4, chi=64 from a common environment: corner
+    singular values and the energy (ipeps.py:1519-1549) against the oracle."""
+    from oracle import ctm as occ
+
+    C = _ctm()
+    A = _c4_tensor()
+    At = C.ipeps_site_tensor(A, symmetric)
+    a = C._build_double_layer_tensor(At)
+    env = C.initialize_ctm_tensor_env(At, 64)
+    with torch.no_grad():
+        for _ in range(3):
+            env = C._ctm_tensor_sweep(env, a, 64, True, "qr")
+        start = [t.todense().cpu().numpy() for t in env]
+        env = C._ctm_tensor_sweep(env, a, 64, True, "qr")
+    oenv = occ.sweep(start, occ.double_layer(A), 64, True, "qr")
+    err = 0.0
+    for t, o in zip(env[:4], oenv[:4]):
+        s = np.sort(np.linalg.svd(t.todense().cpu().numpy(), compute_uv=False))
+        so = np.sort(np.linalg.svd(o, compute_uv=False))
+        err = max(err, float(np.abs(s - so).max() / so.max()))
+    H = occ.heisenberg_gate()
+    e = float(C.compute_energy_ctm_tensor(At, env, torch.as_tensor(H).cuda()))
+    eo = occ.energy(A, oenv, H)
+    print(f"C4 sweep symmetric={symmetric}: corner SV rel err {err:.2e}, E {e:.12f} oracle {eo:.12f}", flush=True)
+    assert err < 1e-10
+    assert abs(e - eo) < 1e-10
+
+
+def test_c4_step_vjp_at_size_matches_frozen_projector_fd(gpu):
+    """The VJP of one CTM step (the building block of the implicit backward,
+    ad_utils.py:722-761) through the block-contraction adjoints at D=4, chi=64,
+    against central differences of the step with its projectors frozen
+    (stop-gradient in the reference) at h=1e-5 and h/2, Richardson-extrapolated
+    (the step's curvature leaves an O(h^2) error of ~3e-5 relative at h=1e-5)."""
+    C = _ctm()
+    At = C.ipeps_site_tensor(_c4_tensor(), True)
+    cfg = C.CTMConfig(chi=64, max_iter=3, conv_tol=0.0, projector_method="qr")
+    env, _ = C._ctm_tensor_fixed_point_impl(At, cfg)
+    a0 = C._leaves([At])[0].detach()
+    e0 = [x.detach() for x in C._leaves(env)]
+    rng = np.random.default_rng(4)
+    ua = torch.as_tensor(rng.standard_normal(a0.shape)).cuda()
+    ue = [torch.as_tensor(rng.standard_normal(x.shape)).cuda() for x in e0]
+    fz = C.FROZEN_PROJECTORS
+
+    def step(av, ev):
+        out = C._ctm_tensor_step(C._with_leaves([At], [av])[0], C.CTMTensorEnv(*C._with_leaves(env, ev)), cfg)
+        return C._leaves(out)
+
+    try:
+        fz.update(mode="record", list=[], i=0)
+        ai = a0.clone().requires_grad_(True)
+        ei = [x.clone().requires_grad_(True) for x in e0]
+        out = step(ai, ei)
+        w = [torch.as_tensor(rng.standard_normal(o.shape)).cuda() for o in out]
+        ga, *ge = torch.autograd.grad(out, [ai] + ei, w, allow_unused=True)
+        ad = float(torch.sum(ga * ua)) + sum(float(torch.sum(g * u)) for g, u in zip(ge, ue) if g is not None)
+        fds = {}
+        for h in (1e-5, 5e-6):
+            vals = []
+            for sgn in (1, -1):
+                fz.update(mode="replay", i=0)
+                with torch.no_grad():
+                    o = step(a0 + sgn * h * ua, [x + sgn * h * u for x, u in zip(e0, ue)])
+                vals.append(sum(float(torch.sum(oi * wi)) for oi, wi in zip(o, w)))
+            fds[h] = (vals[0] - vals[1]) / (2 * h)
+    finally:
+        fz.update(mode=None, list=[], i=0)
+    # Richardson: the O(h^2) term of the central differences cancels
+    fd = (4.0 * fds[5e-6] - fds[1e-5]) / 3.0
+    print(f"C4 step VJP {ad:.12e} frozen-projector FD h=1e-5 {fds[1e-5]:.12e} h=5e-6 {fds[5e-6]:.12e} "
+          f"extrapolated {fd:.12e}", flush=True)
+    assert abs(ad - fd) < 1e-6 * max(1.0, abs(fd)), (ad, fd)
+
+
+def test_d4_fixed_point_gradient_matches_oracle_fd(gpu, monkeypatch):
+    """The full energy gradient at D=4 through the converged CTM fixed point
+    (GMRES on (I - J^T) lambda = g, ad_utils.py:694-764, block-contraction
+    adjoints) against central differences (h=1e-5) of the oracle's converged
+    forward along random directions, at chi=16 where the CTM converges for a
+    near-product tensor.  The reference computes its projectors under
+    stop_gradient (_ctm_tensor.py:233-296), so its gradient is that of the
+    fixed point of the CTM map with the projectors held at their converged
+    values: the oracle's differences are taken on that frozen map (the full
+    forward's differences, which also move the projectors, differ by a few
+    percent here and are printed).  The corner spectra settle after two sweeps
+    while the environment still moves (energy changes at 1e-8 until ~10
+    sweeps), so both forwards run a fixed 25 sweeps (energy stable to 1e-15)
+    instead of the singular-value criterion, and GMRES runs two restart
+    cycles."""
+    from oracle import ctm as occ
+
+    C = _ctm()
+    monkeypatch.setitem(C.GMRES_OPTIONS, "maxiter", 2)
+    A = _near_product(4, 0.05)
+    H = occ.heisenberg_gate()
+    cfg = C.CTMConfig(chi=16, max_iter=25, conv_tol=0.0, projector_method="qr")
+    t = C.ipeps_site_tensor(A, True)
+    leaf = C._leaves([t])[0].detach().clone().requires_grad_(True)
+    e = C.energy_loss(C._with_leaves([t], [leaf])[0], torch.as_tensor(H).cuda(), cfg)
+    (g,) = torch.autograd.grad(e, [leaf])
+    g = g.reshape(A.shape).cpu().numpy()
+    e_o = occ.loss(A, 16, H, 25, 0.0, "qr")
+    print(f"D=4 chi=16 E {float(e):.13f} oracle {e_o:.13f}; GMRES {C.LAST_BACKWARD}", flush=True)
+    assert abs(float(e) - e_o) < 1e-10
+    assert C.LAST_BACKWARD["relative_residual"] < 1e-8
+    rng = np.random.default_rng(2)
+    us = [rng.standard_normal(A.shape) for _ in range(2)]
+    fds = occ.frozen_gradient_fd(A, 16, H, 25, h=1e-5, directions=us)
+    for u, fd in zip(us, fds):
+        ad = float(np.sum(g * u))
+        full = (occ.loss(A + 1e-5 * u, 16, H, 25, 0.0) - occ.loss(A - 1e-5 * u, 16, H, 25, 0.0)) / 2e-5
+        print(f"  directional derivative AD {ad:.10e}, oracle frozen-projector FD {fd:.10e} "
+              f"(full-forward FD {full:.10e})", flush=True)
+        assert abs(ad - fd) < 1e-6 * max(abs(fd), 1e-3)
+
+
+def test_optimize_gs_ad_matches_oracle_fd_adam(gpu, monkeypatch):
+    """optimize_gs_ad (ipeps.py:2370-2434): three Adam steps with the implicit
+    AD gradient against the oracle's Adam restatement driven by central
+    differences (frozen projectors, as the reference's stop_gradient) of its
+    converged forward (D=2, chi=4, near-product tensor)."""
+    from oracle import ctm as occ
+
+    C = _ctm()
+    monkeypatch.setitem(C.GMRES_OPTIONS, "maxiter", 2)
+    A0 = _near_product(2, 0.1, seed=5)
+    H = occ.heisenberg_gate()
+    # fixed 25-sweep forwards (see the D=4 gradient test)
+    cfg = C.iPEPSConfig(max_bond_dim=2, ctm=C.CTMConfig(chi=4, max_iter=25, conv_tol=0.0, projector_method="qr"),
+                        gs_learning_rate=0.05, gs_num_steps=3)
+    Af, _, E = C.optimize_gs_ad(H, C.ipeps_site_tensor(A0, True), cfg)
+    Ao, Eo, es = occ.optimize_gs(A0, 4, H, 0.05, 3, max_iter=25, conv_tol=0.0,
+                                 grad=lambda x: occ.frozen_gradient_fd(x, 4, H, 25, h=1e-5))
+    got = Af.todense().cpu().numpy()
+    print(f"optimize_gs_ad E {E:.12f} oracle {Eo:.12f} (oracle steps {es}); max |dA| "
+          f"{np.abs(got - Ao).max():.2e}", flush=True)
+    assert abs(E - Eo) < 1e-9
+    assert np.abs(got - Ao).max() < 1e-6
+
+
+# ------------------------------------------------------------------ Z2 HOTRG (chi=48)
+C5_TEMPS = [2.0 + 0.6 * k / 63 for k in range(64)]
+
+
+@pytest.mark.parametrize("T", [C5_TEMPS[0], C5_TEMPS[-1]])
+def test_c5_chi48_trajectory_matches_oracle(gpu, T):
+    """Z2 HOTRG at chi=48 for 30 alternating steps (hotrg.py:54-215): at the
+    first seven steps (bond growth 2 -> 48) and at steps 13 and 29 the
+    oracle's step from the same input tensor gives the same log norm; the run's
+    free energy matches Onsager (trg.py:245-291)."""
+    from oracle import hotrg as oh
+    from tenax_amd.algorithms.hotrg import _hotrg_step_horizontal, _hotrg_step_vertical
+    from tenax_amd.algorithms.trg import compute_ising_tensor, ising_free_energy_exact
+
+    beta = 1.0 / T
+    Tt = compute_ising_tensor(beta, symmetric=True)
+    lnz, worst = 0.0, 0.0
+    for step in range(30):
+        f, fo = (_hotrg_step_horizontal, oh.step_horizontal) if step % 2 == 0 else (_hotrg_step_vertical,
+                                                                                  oh.step_vertical)
+        To = to_oracle(Tt) if (step <= 6 or step in (13, 29)) else None
+        Tt, ln = f(Tt, 48)
+        lnz += ln / 2.0 ** (step + 1)
+        if To is not None:
+            _, lno = fo(To, 48, order="chi5")
+            worst = max(worst, abs(ln - lno))
+    assert min(int(ix.dim) for ix in Tt.indices) == 48
+    f_ex = ising_free_energy_exact(beta)
+    rel = abs(-lnz / beta - f_ex) / abs(f_ex)
+    print(f"C5 T={T:.4f}: worst per-step |d ln norm| {worst:.2e}, f rel err vs Onsager {rel:.2e}", flush=True)
+    assert worst < 1e-12
+    assert rel < 1e-2  # the reference's own Onsager tolerances are 0.5-2% (test_hotrg.py:138-199)
+
+
+def test_c5_batched_64_temperatures_match_onsager(gpu):
+    """All 64 temperatures in lockstep (one batched factorisation per step),
+    chi=48, 30 steps: the free energy per site against Onsager at every T,
+    within 1e-2 (the reference's own tolerances are 0.5-2% at chi=16,
+    test_hotrg.py:138-199; the per-step parity tests pin the algorithm)."""
+    import time
+
+    from tenax_amd.algorithms.hotrg import HOTRGConfig, hotrg_batched
+    from tenax_amd.algorithms.trg import compute_ising_tensor, ising_free_energy_exact
+
+    t0 = time.time()
+    lnz = hotrg_batched([compute_ising_tensor(1 / T, symmetric=True) for T in C5_TEMPS],
+                        HOTRGConfig(max_bond_dim=48, num_steps=30))
+    dt = time.time() - t0
+    rel = []
+    for T, l in zip(C5_TEMPS, lnz):
+        ex = ising_free_energy_exact(1 / T)
+        rel.append(abs(-l * T - ex) / abs(ex))
+        assert rel[-1] < 1e-2, (T, rel[-1])
+    print(f"C5 64 temperatures: {dt:.1f}s, max rel err vs Onsager {max(rel):.2e}", flush=True)
