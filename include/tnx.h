@@ -162,6 +162,29 @@ int tnx_transpose_execute(const tnx_plan *plan, const double *a, double *out,
 int tnx_plan_transpose_graded(const tnx_blocks *a, const tnx_leg *legs, const int32_t *perm,
                               int32_t graded, void *stream, tnx_plan **plan);
 
+/* Work-list validation (no reference counterpart; a debug and test aid).
+ * tnx_plan_validate checks every grouped-GEMM work list of a plan (pairwise,
+ * matvec or environment; default and tiled task lists) and the matvec's mix
+ * lists against the block layouts of the operands and intermediates: every
+ * range a launch can address lies inside one block of its operand, every task
+ * is a tile of its output with a non-empty K range, every split-K slot lies in
+ * the workspace.  TNX_ERR_BAD_STRUCTURE with the first violation otherwise.
+ * With TNX_VALIDATE=1 in the environment every plan is validated when it is
+ * built and when its tiling changes. */
+int tnx_plan_validate(const tnx_plan *plan, void *stream);
+/* The same check on host arrays (no device work): pairs / outs / tasks are
+ * n records of the plan's internal layout (32 bytes each):
+ *   pair {int64 a_off, b_off; int32 K, lda, ldb, neg}
+ *   out  {int64 c_off; int32 M, N, ldc, pair_begin, pair_end, pad}
+ *   task {int32 out, m0, n0, kt0, kt1, slot, pad0, pad1}
+ * a_layout 0: A blocks M x K (MK), 1: K x M (KM); B is K x N.  *_blocks are
+ * n pairs {offset, size} (elements) of each operand's blocks; bm x bn is the
+ * task tile and ws_tiles the split-K workspace size in tiles. */
+int tnx_check_gemm_worklist(int32_t a_layout, const void *pairs, int32_t n_pairs, const void *outs,
+                            int32_t n_outs, const void *tasks, int32_t n_tasks, int32_t bm, int32_t bn,
+                            int64_t ws_tiles, const int64_t *a_blocks, int32_t n_a, const int64_t *b_blocks,
+                            int32_t n_b, const int64_t *c_blocks, int32_t n_c);
+
 #ifdef __cplusplus
 }
 #endif

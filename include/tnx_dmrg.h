@@ -124,7 +124,13 @@ int tnx_lanczos_noalloc(const tnx_plan *matvec_plan, const double *L, const doub
  *   result   synchronises the stream and returns energy and step count
  *            (theta then holds the eigenvector);
  *   free     releases the graph and its workspace.
- * The plan and the operand buffers must outlive the graph. */
+ * The operand buffers must outlive the graph, and so must the plan:
+ * tnx_plan_free refuses (TNX_ERR_ARG) a plan with graphs not yet freed.  The
+ * graph has its own Krylov workspace but shares the plan's intermediates
+ * (matvec stages, split-K workspace) with eager tnx_matvec_execute /
+ * tnx_lanczos calls on the same plan: replaying a graph while an eager call
+ * on the same plan runs on another stream races, so order them on one stream
+ * (or synchronise between them). */
 typedef struct tnx_graph_s tnx_graph;
 int tnx_lanczos_graph_create(const tnx_plan *matvec_plan, const double *L, const double *Wl,
                              const double *Wr, const double *R, double *theta, int32_t max_iter,

@@ -14,7 +14,9 @@ import threading
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libtnx.so")
+# TNX_LIB_PATH selects another build of the library (e.g. a variant compiled
+# with different kernel parameters); the in-tree libtnx.so otherwise.
+LIB_PATH = os.environ.get("TNX_LIB_PATH") or os.path.join(_HERE, "libtnx.so")
 
 TNX_OK = 0
 TNX_ERR_BAD_STRUCTURE = -1
@@ -87,6 +89,9 @@ SIGNATURES = {
     "tnx_contract_execute_perblock": (C.c_int, [_p, _p, _p, _p, _i32, _p]),
     "tnx_plan_transpose": (C.c_int, [C.POINTER(Blocks), C.POINTER(Leg), _PI32, _p, C.POINTER(_p)]),
     "tnx_transpose_execute": (C.c_int, [_p, _p, _p, _p]),
+    "tnx_plan_validate": (C.c_int, [_p, _p]),
+    "tnx_check_gemm_worklist": (C.c_int, [_i32, _p, _i32, _p, _i32, _p, _i32, _i32, _i32, _i64, _PI64, _i32,
+                                          _PI64, _i32, _PI64, _i32]),
     "tnx_plan_transpose_graded": (C.c_int, [C.POINTER(Blocks), C.POINTER(Leg), _PI32, _i32, _p, C.POINTER(_p)]),
     # include/tnx_dmrg.h
     "tnx_matvec_plan": (C.c_int, [C.POINTER(Blocks)] * 5 + [_p, C.POINTER(_p)]),

@@ -43,6 +43,18 @@ def clear_plan_cache():
         _fused.clear()
 
 
+def live_pairwise_plans():
+    """The cached pairwise contraction plans (most recently used last)."""
+    with _lock:
+        return [v[0] for k, v in _cache.items() if k[0] == "pair"]
+
+
+def live_env_plans():
+    """The cached environment-update plans (most recently used last)."""
+    with _lock:
+        return [v[0] for k, v in _fused.items() if k[0] == "env"]
+
+
 def _struct_sig(t):
     return (t.keys.tobytes(), t.shapes.tobytes(), t.keys.shape)
 

@@ -269,6 +269,22 @@ __device__ __forceinline__ int mix_nj(int D0, int D1, int D2, int ca) {
   return ca == 0 ? D0 : (ca == 1 ? D1 : D2);
 }
 
+// Row pitch (elements) of a mix item's block, which is the A operand (M x K,
+// row-major) of the stage-3 GEMM: K (D1*D2 for ca == 2, D2 for ca == 1)
+// rounded up to 16 doubles, so that with 128-byte aligned item offsets every
+// 16-wide K tile of a row is one aligned 128-byte line (an unaligned row
+// segment touches two).  The padding elements are never written or read.
+constexpr int64_t Y_ALIGN = 16;
+__host__ __device__ __forceinline__ int64_t item_pitch(int D1, int D2, int ca) {
+  if (ca == 0) return (int64_t)D2;
+  const int64_t k = ca == 2 ? (int64_t)D1 * D2 : (int64_t)D2;
+  return (k + Y_ALIGN - 1) / Y_ALIGN * Y_ALIGN;
+}
+__host__ __device__ __forceinline__ int64_t item_size(int D0, int D1, int D2, int ca) {
+  if (ca == 0) return (int64_t)D0 * D1 * D2;
+  return (ca == 2 ? (int64_t)D0 : (int64_t)D0 * D1) * item_pitch(D1, D2, ca);
+}
+
 // Item fields are read one by one (uniform across the workgroup, so they go
 // through scalar loads; a copied struct whose field is picked by a run-time
 // axis ends up in LDS).  The item's (term, b) products come as a flat entry
@@ -287,7 +303,9 @@ __global__ void __launch_bounds__(MIX_WG) mix_kernel(const MixItem *__restrict__
                                                   const int64_t *__restrict__ task_start,
                                                   const int32_t *__restrict__ item_entry,
                                                   const MixEntry *__restrict__ ents, const double *__restrict__ coef,
-                                                  const double *__restrict__ X, double *__restrict__ Y) {
+                                                  const double *__restrict__ X, double *__restrict__ Y,
+                                                  const int32_t *__restrict__ stop) {
+  if (stop && *stop) return;  // a Lanczos solve that has ended: no work
   const int32_t item = task_item[blockIdx.x];
   if (item < 0) return;  // padding of the fused launch order
   const MixItem *ip = items + item;
@@ -355,8 +373,8 @@ __global__ void __launch_bounds__(MIX_WG) mix_kernel(const MixItem *__restrict__
     const int64_t uv = uv0 + (int64_t)r * MIX_WG;
     int64_t base, sj;
     if (ca == 0) { base = uv; sj = nuv; }
-    else if (ca == 1) { base = (int64_t)us[r] * D1 * D2 + vs[r]; sj = D2; }
-    else { base = uv * D2; sj = 1; }
+    else if (ca == 1) { const int64_t P = item_pitch(D1, D2, 1); base = (int64_t)us[r] * D1 * P + vs[r]; sj = P; }
+    else { base = (int64_t)us[r] * item_pitch(D1, D2, 2) + (int64_t)vs[r] * D2; sj = 1; }
     double *yp = Y + ip->y_off + base + (int64_t)j0 * sj;
 #pragma unroll
     for (int q = 0; q < MIX_NJ; ++q)
@@ -456,7 +474,7 @@ __global__ void item_sizes_kernel(int64_t nt, ItemSpec S, const TermRec *__restr
   const TermRec tr = terms[sids[k]];
   int D0, D1, D2, ca;
   item_dims(S, tr.out, tr.partner, D0, D1, D2, ca);
-  isize[k] = (int64_t)D0 * D1 * D2;
+  isize[k] = item_size(D0, D1, D2, ca);
 }
 
 __global__ void build_items_kernel(int64_t nt, ItemSpec S, const TermRec *__restrict__ terms,
@@ -540,11 +558,11 @@ __global__ void build_stage2_kernel(int64_t ni, ItemSpec S, const MixItem *__res
   g.b_off = S.Part.o[r];
   if (S.kind == 0) {
     g.K = it.D1 * it.D2;  // (d, f)
-    g.lda = g.K;
+    g.lda = (int32_t)item_pitch(it.D1, it.D2, it.ca);
     g.ldb = S.Part.dim(r, 2);  // g
   } else {
     g.K = it.D2;  // c (x dim 1)
-    g.lda = g.K;
+    g.lda = (int32_t)item_pitch(it.D1, it.D2, it.ca);
     g.ldb = S.Part.dim(r, 2);  // f
   }
   pairs[k] = g;
@@ -661,6 +679,7 @@ struct FusedPlan : public tnx_plan_s {
   std::vector<int32_t> pb2_off, pb2_cnt;
   std::vector<uint8_t> pb2_first;
   std::vector<GemmOut> pb2_zero;  // outs without pairs: zero-filled
+  Extents e1, e2, epart;  // block layouts of the stage-1 operands and the stage-3 B operand (validation)
 };
 
 // Launch order of the fused mix.  An X block (one stage-1 pair) feeds several
@@ -763,6 +782,9 @@ static void build_fused(FusedPlan &P, int kind, DevTable &P1, DevTable &P2, DevT
                         DevTable &Part, DevTable *OutT, const tnx_leg *out_legs, hipStream_t s) {
   P.fkind = kind;
   P.kind = kind == 0 ? PlanKind::Matvec : PlanKind::EnvLeft;
+  P.e1 = extents_of(P1.hs.data(), P1.ho.data(), P1.n, P1.rank);
+  P.e2 = extents_of(P2.hs.data(), P2.ho.data(), P2.n, P2.rank);
+  P.epart = extents_of(Part.hs.data(), Part.ho.data(), Part.n, Part.rank);
   DevBuf<char> tmp;
   KeySet V;
   TView outv{};
@@ -992,12 +1014,62 @@ static void build_fused(FusedPlan &P, int kind, DevTable &P1, DevTable &P2, DevT
   P.bytes = 8.0 * (double)(in_bytes + P.out_total);
   P.n_pairs = P.s1.n_pairs + P.s2.n_pairs;
   P.n_tasks = P.s1.n_tasks + P.s2.n_tasks + P.n_mix_tasks;
+  if (validation_enabled()) validate_fused(&P, s);
+}
+
+void validate_fused(const tnx_plan_s *plan, hipStream_t s) {
+  const FusedPlan &P = static_cast<const FusedPlan &>(*plan);
+  const std::string what = P.fkind == 0 ? "matvec plan" : "environment plan";
+  auto fail = [&](const std::string &m) { throw Error{TNX_ERR_BAD_STRUCTURE, what + ": " + m}; };
+  // X: the stage-1 output blocks, inside the X buffer and disjoint
+  Extents ex;
+  for (const GemmOut &o : P.s1.h_outs)
+    if (o.M > 0 && o.N > 0) ex.add(o.c_off, (int64_t)(o.M - 1) * o.ldc + o.N);
+  ex.finish();
+  if (ex.total > P.x_total) fail("stage-1 outputs reach past the X buffer");
+  check_program(P.s1, P.e1, P.e2, ex, what + " stage 1", s);
+  // Y: the mix items, inside the Y buffer and disjoint
+  std::vector<MixItem> items(P.n_items);
+  std::vector<int32_t> ie(P.n_items + 1, 0);
+  if (P.n_items) {
+    P.items.download(items.data(), P.n_items, s);
+    P.item_entry.download(ie.data(), P.n_items + 1, s);
+  }
+  TNX_CHECK_HIP(hipStreamSynchronize(s));
+  const int64_t ne = P.n_items ? ie[P.n_items] : 0;
+  std::vector<MixEntry> ents(std::max<int64_t>(ne, 0));
+  if (ne > 0) P.ents.download(ents.data(), ne, s);
+  TNX_CHECK_HIP(hipStreamSynchronize(s));
+  Extents ey;
+  for (int64_t i = 0; i < P.n_items; ++i) {
+    const MixItem &it = items[i];
+    if (it.D0 <= 0 || it.D1 <= 0 || it.D2 <= 0) fail("mix item " + std::to_string(i) + " has an empty extent");
+    ey.add(it.y_off, item_size(it.D0, it.D1, it.D2, it.ca));
+    // (u, v) ranges of the item's X reads (see mix_kernel)
+    const int64_t nj = it.ca == 0 ? it.D0 : (it.ca == 1 ? it.D1 : it.D2);
+    const int64_t dv = it.ca == 2 ? it.D1 : it.D2;
+    const int64_t nu = (int64_t)it.D0 * it.D1 * it.D2 / nj / dv;
+    if (ie[i] < 0 || ie[i + 1] < ie[i] || ie[i + 1] > ne) fail("mix item " + std::to_string(i) + " entry range");
+    for (int32_t e = ie[i]; e < ie[i + 1]; ++e) {
+      const MixEntry &m = ents[e];
+      if (m.su < 0 || m.sv < 0) fail("mix entry " + std::to_string(e) + " has a negative stride");
+      if (!ex.covers(m.x_off, m.x_off + (nu - 1) * m.su + (dv - 1) * m.sv + 1))
+        fail("mix entry " + std::to_string(e) + " reads outside one X block");
+      if (m.coef_off < 0 || m.coef_off + nj > P.coef_total)
+        fail("mix entry " + std::to_string(e) + " reads past the coefficients");
+    }
+  }
+  ey.finish();
+  if (ey.total > P.y_total) fail("mix items reach past the Y buffer");
+  const Extents eout = extents_of(P.out_shapes.data(), P.out_offsets.data(), P.out_nblocks, P.out_rank);
+  check_program(P.s2, ey, P.epart, eout, what + " stage 3", s);
 }
 
 // coef_ready: the MPO coefficients of these W1/W2 are already in P.coef (the
 // Lanczos iteration applies one operator many times).
 static void run_fused(const FusedPlan &P, const double *s1A, const double *s1B, const double *W1, const double *W2,
-                      const double *s2B, double *out, int compute, hipStream_t s, bool coef_ready = false) {
+                      const double *s2B, double *out, int compute, hipStream_t s, bool coef_ready = false,
+                      const int32_t *stop = nullptr) {
   if (P.n_terms == 0) {
     if (P.out_total > 0)
       hipLaunchKernelGGL(zero_kernel, dim3(nb256(P.out_total)), dim3(256), 0, s, P.out_total, out);
@@ -1006,10 +1078,10 @@ static void run_fused(const FusedPlan &P, const double *s1A, const double *s1B, 
   }
   if (!coef_ready)
     launch_coef(P.n_terms, P.cspec.p, W1, W2, P.coef.p, s);
-  launch_grouped_gemm(P.s1, s1A, s1B, P.X.p, compute, false, s);
+  launch_grouped_gemm(P.s1, s1A, s1B, P.X.p, compute, false, s, true, stop);
   hipLaunchKernelGGL(mix_kernel, dim3(P.n_mixf_tasks), dim3(MIX_WG), 0, s, P.items.p, P.mixf_titem.p, P.mixf_tstart.p,
-                     P.item_entry.p, P.ents.p, P.coef.p, P.X.p, P.Y.p);
-  launch_grouped_gemm(P.s2, P.Y.p, s2B, out, compute, false, s);
+                     P.item_entry.p, P.ents.p, P.coef.p, P.X.p, P.Y.p, stop);
+  launch_grouped_gemm(P.s2, P.Y.p, s2B, out, compute, false, s, true, stop);
   TNX_CHECK_HIP(hipGetLastError());
 }
 
@@ -1071,7 +1143,7 @@ static void run_fused_perblock(FusedPlan &P, const double *s1A, const double *s1
                     P.s1.pairs.p, s1A, s1B, P.X.p, false);
   for (int64_t k = 0; k < P.n_items; ++k)
     hipLaunchKernelGGL(mix_kernel, dim3(P.pbm_cnt[k]), dim3(MIX_WG), 0, s, P.items.p, P.mix_titem.p + P.pbm_off[k],
-                       P.mix_tstart.p + P.pbm_off[k], P.item_entry.p, P.ents.p, P.coef.p, P.X.p, P.Y.p);
+                       P.mix_tstart.p + P.pbm_off[k], P.item_entry.p, P.ents.p, P.coef.p, P.X.p, P.Y.p, (const int32_t *)nullptr);
   for (const auto &z : P.pb2_zero) {
     const int64_t n = (int64_t)z.M * z.N;
     if (n) hipLaunchKernelGGL(zero_kernel, dim3(nb256(n)), dim3(256), 0, s, n, out + z.c_off);
@@ -1530,7 +1602,8 @@ static void lanczos_enqueue(const FusedPlan &P, const double *L, const double *W
     double *v = V + (int64_t)k * n;
     double *w = V + (int64_t)(k + 1) * n;
     const double *vp = k > 0 ? V + (int64_t)(k - 1) * n : v;
-    run_fused(P, L, v, Wl, Wr, R, w, compute, s, k > 0);
+    // after a breakdown step the matvecs of the remaining steps do no work
+    run_fused(P, L, v, Wl, Wr, R, w, compute, s, k > 0, &W.st->stopped);
     hipLaunchKernelGGL(dot_partial_kernel, rg, vb, 0, s, v, w, n, part, W.st);
     double *npart = part;  // partials of w.w
     if (reorth) {
@@ -1702,7 +1775,7 @@ int tnx_matvec_execute_stage(const tnx_plan *plan, int32_t stage, const double *
         break;
       case 2:
         hipLaunchKernelGGL(mix_kernel, dim3(P.n_mixf_tasks), dim3(MIX_WG), 0, s, P.items.p, P.mixf_titem.p,
-                           P.mixf_tstart.p, P.item_entry.p, P.ents.p, P.coef.p, P.X.p, P.Y.p);
+                           P.mixf_tstart.p, P.item_entry.p, P.ents.p, P.coef.p, P.X.p, P.Y.p, (const int32_t *)nullptr);
         break;
       case 3:
         launch_grouped_gemm(P.s2, P.Y.p, R, y, compute, false, s, !gemm_only);
@@ -1819,6 +1892,7 @@ struct tnx_graph_s {
   ~tnx_graph_s() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (plan) plan->live_graphs.fetch_sub(1);
   }
 };
 
@@ -1864,6 +1938,7 @@ int tnx_lanczos_graph_create(const tnx_plan *matvec_plan, const double *L, const
     const FusedPlan &P = static_cast<const FusedPlan &>(*matvec_plan);
     auto G = std::make_unique<tnx_graph_s>();
     G->plan = &P;
+    P.live_graphs.fetch_add(1);
     G->gen = P.generation;
     G->max_iter = max_iter;
     G->tol = tol;

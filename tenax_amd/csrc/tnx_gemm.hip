@@ -249,7 +249,7 @@ __global__ __launch_bounds__(GEMM_THREADS) __attribute__((amdgpu_waves_per_eu(
     (COMPUTE == 2 && BM * BN > 8192) ? 1 : 2))) void grouped_gemm_kernel(
     const GemmTask *__restrict__ tasks, const GemmOut *__restrict__ outs,
     const GemmPair *__restrict__ pairs, const double *__restrict__ A, const double *__restrict__ B,
-    double *__restrict__ C, double *__restrict__ ws, int accumulate) {
+    double *__restrict__ C, double *__restrict__ ws, int accumulate, const int32_t *__restrict__ stop) {
   constexpr int NT = (COMPUTE == 2) ? 2 : 1;  // hi/lo planes for split
   constexpr int AP = BM + 1, BP = BN + 1;
   constexpr int TI = BM / 32, TJ = BN / 32;
@@ -259,6 +259,8 @@ __global__ __launch_bounds__(GEMM_THREADS) __attribute__((amdgpu_waves_per_eu(
   __shared__ T As[2][NT][GEMM_BK * AP];
   __shared__ T Bs[2][NT][GEMM_BK * BP];
 
+  // a device-side stop flag (a Lanczos solve that has ended): no work
+  if (stop && *stop) return;
   const GemmTask t = tasks[blockIdx.x];
   if (t.out < 0) return;  // padding entry (uniform over the workgroup)
   const GemmOut o = outs[t.out];
@@ -423,27 +425,29 @@ __global__ __launch_bounds__(GEMM_THREADS) __attribute__((amdgpu_waves_per_eu(
   // nops): 3% faster on an MK launch, but 3% slower on a KM one, which keeps
   // the single guarded loop.  Both versions pass the same barriers, so the
   // waves of a workgroup may take different ones.
+  // step(i, X, Y): tile i is computed from LDS, X holds tile i+1 (stored
+  // after the compute), Y (whose tile is already in LDS) receives the next
+  // prefetch.  The prefetch is issued unconditionally (past the last tile it
+  // re-reads the last pair's K tiles and is never stored), so the number of
+  // loads in flight is the same on every path.
+  auto step = [&](int i, Stage<BM, BN> &X, Stage<BM, BN> &Y, auto guard) {
+    load(Y);
+    compute_tile(i & 1, guard);
+    if (i + 1 < n) {
+      stage_store((i + 1) & 1, X);
+      __syncthreads();
+    }
+  };
   auto main_loop = [&](auto guard) {
+    // two register stages: tile i+1 waits in registers while tile i+2 is in flight
     Stage<BM, BN> r0, r1;
     load(r0);
     load(r1);
     stage_store(0, r0);
     __syncthreads();
-    // step(i, X, Y): X holds tile i+1 (to be stored), Y receives tile i+2.
-    // The prefetch is issued unconditionally (past the last tile it re-reads
-    // the last pair's first K tile and is never stored), so the number of
-    // loads in flight is the same on every path.
-    auto step = [&](int i, Stage<BM, BN> &X, Stage<BM, BN> &Y) {
-      load(Y);
-      compute_tile(i & 1, guard);
-      if (i + 1 < n) {
-        stage_store((i + 1) & 1, X);
-        __syncthreads();
-      }
-    };
     for (int i = 0; i < n; i += 2) {
-      step(i, r1, r0);
-      if (i + 1 < n) step(i + 1, r0, r1);
+      step(i, r1, r0, guard);
+      if (i + 1 < n) step(i + 1, r0, r1, guard);
     }
   };
   if (n > 0) {
@@ -486,7 +490,8 @@ template <int BM, int BN>
 __global__ __launch_bounds__(256) void split_reduce_kernel(const SplitTile *__restrict__ red,
                                                            const GemmOut *__restrict__ outs,
                                                            const double *__restrict__ ws, double *__restrict__ C,
-                                                           int accumulate) {
+                                                           int accumulate, const int32_t *__restrict__ stop) {
+  if (stop && *stop) return;
   constexpr int PER = BM * BN / 256;
   const SplitTile r = red[blockIdx.x];
   const GemmOut o = outs[r.out];
@@ -513,19 +518,20 @@ __global__ __launch_bounds__(256) void split_reduce_kernel(const SplitTile *__re
 
 template <int ALAY, int BM, int BN, bool SIGNED>
 static void launch_tile(int compute, dim3 grid, hipStream_t s, const GemmTask *tasks, const GemmOut *outs,
-                        const GemmPair *pairs, const double *A, const double *B, double *C, double *ws, int acc) {
+                        const GemmPair *pairs, const double *A, const double *B, double *C, double *ws, int acc,
+                        const int32_t *stop) {
   switch (compute) {
     case TNX_COMPUTE_F64:
       hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, 0, BM, BN, SIGNED>), grid, dim3(GEMM_THREADS), 0, s, tasks, outs, pairs,
-                         A, B, C, ws, acc);
+                         A, B, C, ws, acc, stop);
       break;
     case TNX_COMPUTE_F32:
       hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, 1, BM, BN, SIGNED>), grid, dim3(GEMM_THREADS), 0, s, tasks, outs, pairs,
-                         A, B, C, ws, acc);
+                         A, B, C, ws, acc, stop);
       break;
     case TNX_COMPUTE_SPLIT_F32:
       hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, 2, BM, BN, SIGNED>), grid, dim3(GEMM_THREADS), 0, s, tasks, outs, pairs,
-                         A, B, C, ws, acc);
+                         A, B, C, ws, acc, stop);
       break;
     default:
       throw Error{TNX_ERR_DTYPE, "unknown compute mode " + std::to_string(compute)};
@@ -535,53 +541,53 @@ static void launch_tile(int compute, dim3 grid, hipStream_t s, const GemmTask *t
 template <int ALAY, bool SIGNED>
 static void launch_shape(int shape, int compute, dim3 grid, hipStream_t s, const GemmTask *tasks,
                          const GemmOut *outs, const GemmPair *pairs, const double *A, const double *B, double *C,
-                         double *ws, int acc) {
+                         double *ws, int acc, const int32_t *stop) {
   switch (shape) {
-    case 0: launch_tile<ALAY, 64, 64, SIGNED>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc); break;
-    case 1: launch_tile<ALAY, 128, 64, SIGNED>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc); break;
-    case 2: launch_tile<ALAY, 64, 128, SIGNED>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc); break;
-    case 3: launch_tile<ALAY, 128, 128, SIGNED>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc); break;
+    case 0: launch_tile<ALAY, 64, 64, SIGNED>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop); break;
+    case 1: launch_tile<ALAY, 128, 64, SIGNED>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop); break;
+    case 2: launch_tile<ALAY, 64, 128, SIGNED>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop); break;
+    case 3: launch_tile<ALAY, 128, 128, SIGNED>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop); break;
     default: throw Error{TNX_ERR_ARG, "unknown tile shape " + std::to_string(shape)};
   }
 }
 
 static void launch_any(int a_lay, int b_lay, bool signs, int shape, int compute, int n_tasks, hipStream_t s,
                        const GemmTask *tasks, const GemmOut *outs, const GemmPair *pairs, const double *A,
-                       const double *B, double *C, double *ws, int acc) {
+                       const double *B, double *C, double *ws, int acc, const int32_t *stop) {
   if (n_tasks <= 0) return;
   TNX_REQUIRE(b_lay == LAY_KN, TNX_ERR_ARG, "B operands must be K x N row-major");
   dim3 grid(n_tasks);
   if (a_lay == LAY_MK) {
     if (signs)
-      launch_shape<LAY_MK, true>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc);
+      launch_shape<LAY_MK, true>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop);
     else
-      launch_shape<LAY_MK, false>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc);
+      launch_shape<LAY_MK, false>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop);
   } else {
     if (signs)
-      launch_shape<LAY_KM, true>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc);
+      launch_shape<LAY_KM, true>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop);
     else
-      launch_shape<LAY_KM, false>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc);
+      launch_shape<LAY_KM, false>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop);
   }
   TNX_CHECK_HIP(hipGetLastError());
 }
 
 void launch_grouped_gemm(const GemmProgram &prog, const double *A, const double *B, double *C, int compute,
-                         bool accumulate, hipStream_t s, bool reduce) {
+                         bool accumulate, hipStream_t s, bool reduce, const int32_t *stop) {
   const int acc = accumulate ? 1 : 0;
   if (!prog.tiled) {
     launch_any(prog.a_lay, prog.b_lay, prog.signs, 0, compute, prog.n_tasks, s, prog.tasks.p, prog.outs.p, prog.pairs.p, A, B,
-               C, nullptr, acc);
+               C, nullptr, acc, stop);
     return;
   }
   launch_any(prog.a_lay, prog.b_lay, prog.signs, prog.cfg.shape, compute, prog.n_vtasks, s, prog.vtasks.p, prog.outs.p,
-             prog.pairs.p, A, B, C, prog.ws.p, acc);
+             prog.pairs.p, A, B, C, prog.ws.p, acc, stop);
   if (reduce && prog.n_red > 0) {
     const dim3 g(prog.n_red), b(256);
     switch (prog.cfg.shape) {
-      case 0: hipLaunchKernelGGL((split_reduce_kernel<64, 64>), g, b, 0, s, prog.red.p, prog.outs.p, prog.ws.p, C, acc); break;
-      case 1: hipLaunchKernelGGL((split_reduce_kernel<128, 64>), g, b, 0, s, prog.red.p, prog.outs.p, prog.ws.p, C, acc); break;
-      case 2: hipLaunchKernelGGL((split_reduce_kernel<64, 128>), g, b, 0, s, prog.red.p, prog.outs.p, prog.ws.p, C, acc); break;
-      default: hipLaunchKernelGGL((split_reduce_kernel<128, 128>), g, b, 0, s, prog.red.p, prog.outs.p, prog.ws.p, C, acc); break;
+      case 0: hipLaunchKernelGGL((split_reduce_kernel<64, 64>), g, b, 0, s, prog.red.p, prog.outs.p, prog.ws.p, C, acc, stop); break;
+      case 1: hipLaunchKernelGGL((split_reduce_kernel<128, 64>), g, b, 0, s, prog.red.p, prog.outs.p, prog.ws.p, C, acc, stop); break;
+      case 2: hipLaunchKernelGGL((split_reduce_kernel<64, 128>), g, b, 0, s, prog.red.p, prog.outs.p, prog.ws.p, C, acc, stop); break;
+      default: hipLaunchKernelGGL((split_reduce_kernel<128, 128>), g, b, 0, s, prog.red.p, prog.outs.p, prog.ws.p, C, acc, stop); break;
     }
     TNX_CHECK_HIP(hipGetLastError());
   }
@@ -591,7 +597,8 @@ void launch_grouped_gemm(const GemmProgram &prog, const double *A, const double 
 void launch_prepared(int a_lay, int b_lay, bool signs, int compute, int n_tasks, hipStream_t s,
                      const GemmTask *tasks, const GemmOut *outs, const GemmPair *pairs, const double *A,
                      const double *B, double *C, bool accumulate) {
-  launch_any(a_lay, b_lay, signs, 0, compute, n_tasks, s, tasks, outs, pairs, A, B, C, nullptr, accumulate ? 1 : 0);
+  launch_any(a_lay, b_lay, signs, 0, compute, n_tasks, s, tasks, outs, pairs, A, B, C, nullptr, accumulate ? 1 : 0,
+             nullptr);
 }
 
 }  // namespace tnx

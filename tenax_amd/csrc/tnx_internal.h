@@ -3,6 +3,7 @@
 #include <cstring>
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <memory>
 #include <stdexcept>
@@ -186,8 +187,11 @@ void set_tiling(GemmProgram &prog, const TileCfg &cfg, hipStream_t s);
 
 // Launch the grouped GEMM: C[outs] (=|+=) sum_pairs A*B.
 // reduce = false skips the split-K reduction (profiling of the GEMM kernel alone).
+// stop: optional device flag; when it reads non-zero the launch does nothing
+// (the matvecs of a Lanczos solve after its breakdown step).
 void launch_grouped_gemm(const GemmProgram &prog, const double *A, const double *B, double *C,
-                         int compute, bool accumulate, hipStream_t s, bool reduce = true);
+                         int compute, bool accumulate, hipStream_t s, bool reduce = true,
+                         const int32_t *stop = nullptr);
 // Launch a prepared program slice (tasks/outs/pairs already on the device).
 void launch_prepared(int a_lay, int b_lay, bool signs, int compute, int n_tasks, hipStream_t s,
                      const GemmTask *tasks, const GemmOut *outs, const GemmPair *pairs, const double *A,
@@ -225,6 +229,9 @@ struct tnx_plan_base {
   // bumped whenever the device work lists are rebuilt (tnx_plan_set_tiling);
   // captured graphs refuse to replay work lists of an older generation
   uint64_t generation = 0;
+  // captured graphs recorded on this plan and not yet freed; the plan
+  // refuses to be freed while any remain (they launch its work lists)
+  mutable std::atomic<int32_t> live_graphs{0};
 };
 
 }  // namespace tnx
@@ -234,4 +241,42 @@ struct tnx_plan_s : public tnx::tnx_plan_base {};
 namespace tnx {
 // The grouped-GEMM program of a fused (matvec / environment) plan's stage 1 or 3.
 GemmProgram *fused_program(tnx_plan_s *plan, int stage);
+}  // namespace tnx
+
+namespace tnx {
+// ---------------------------------------------------------------- work-list validation
+// Block layout of one operand for validation: half-open element ranges
+// [off, off + size) of its blocks (any order; zero-size blocks ignored).
+struct Extents {
+  std::vector<int64_t> lo, hi;  // sorted by lo after finish()
+  int64_t total = 0;            // buffer size in elements
+  void add(int64_t off, int64_t size);
+  void finish();  // sorts, and checks that blocks do not overlap
+  // true when [a, b) lies inside one block
+  bool covers(int64_t a, int64_t b) const;
+};
+// Extents of a packed block table (shapes n x rank, offsets n).
+Extents extents_of(const int32_t *shapes, const int64_t *offsets, int32_t n, int32_t rank);
+
+// Checks a grouped-GEMM work list against its operands' block layouts: every
+// output's C range inside one C block, every K-segment's A and B range inside
+// one block of its operand (A as M x K rows of pitch lda for the MK layout,
+// K x M for KM; B as K x N), row pitches at least the row length, pair ranges
+// inside the pair list, and every task on an existing tile of its output with
+// a non-empty K range and a workspace slot inside the workspace.  bm x bn is
+// the task tile; ws_tiles the number of workspace slots (0 without split K).
+// Throws Error{TNX_ERR_BAD_STRUCTURE, what + detail} on the first violation.
+void check_worklist(int a_lay, const GemmPair *pairs, int64_t n_pairs, const GemmOut *outs, int64_t n_outs,
+                    const GemmTask *tasks, int64_t n_tasks, int bm, int bn, int64_t ws_tiles, const Extents &A,
+                    const Extents &B, const Extents &C, const std::string &what);
+// check_worklist over a finalized program: its device-built default task list
+// and, when a tiling is set, the tiled list.  Downloads what it needs.
+void check_program(const GemmProgram &prog, const Extents &A, const Extents &B, const Extents &C,
+                   const std::string &what, hipStream_t s);
+// TNX_VALIDATE set (and not "0") in the environment: every plan is checked
+// when it is built and when its tiling changes.
+bool validation_enabled();
+// Both GEMM programs and the mix lists of a matvec / environment plan
+// against the block layouts of its operands and intermediates.
+void validate_fused(const tnx_plan_s *plan, hipStream_t s);
 }  // namespace tnx

@@ -597,7 +597,14 @@ struct PairwisePlan : public tnx_plan_s {
   std::vector<uint8_t> pb_first;
   bool pb_ready = false;
   int64_t a_total = 0, b_total = 0;
+  Extents ea, eb;  // operand block layouts (work-list validation)
 };
+
+// A pairwise plan's program against its operands' and its output's block layouts.
+static void validate_pairwise(const PairwisePlan &P, hipStream_t s) {
+  const Extents ec = extents_of(P.out_shapes.data(), P.out_offsets.data(), P.out_nblocks, P.out_rank);
+  check_program(P.prog, P.ea, P.eb, ec, "pairwise plan", s);
+}
 
 struct TDesc {
   int64_t in_off, out_off, size;
@@ -630,6 +637,8 @@ static void build_pairwise(PairwisePlan &P, const tnx_blocks *a, const tnx_block
   P.prog.a_lay = a_kfirst ? LAY_KM : LAY_MK;
   TNX_REQUIRE(b_kfirst == 1, TNX_ERR_ARG, "B operand legs must be [contracted..., free...] (b_kfirst = 1)");
   P.prog.b_lay = LAY_KN;
+  P.ea = extents_of(a->shapes, a->offsets, a->n_blocks, a->rank);
+  P.eb = extents_of(b->shapes, b->offsets, b->n_blocks, b->rank);
   for (int i = 0; i < a->n_blocks; ++i) {
     int64_t sz = 1;
     for (int l = 0; l < a->rank; ++l) sz *= a->shapes[i * a->rank + l];
@@ -747,6 +756,7 @@ static void build_pairwise(PairwisePlan &P, const tnx_blocks *a, const tnx_block
   P.bytes = 8.0 * (double)(P.a_total + P.b_total + P.out_total);
   P.n_pairs = P.prog.n_pairs;
   P.n_tasks = P.prog.n_tasks;
+  if (validation_enabled()) validate_pairwise(P, s);
 }
 
 static void ensure_perblock(PairwisePlan &P, hipStream_t s) {
@@ -993,7 +1003,22 @@ int tnx_plan_stats(const tnx_plan *plan, double *flops, double *bytes, int32_t *
 }
 
 int tnx_plan_free(tnx_plan *plan) {
-  return guarded([&] { delete plan; });
+  return guarded([&] {
+    if (plan)
+      TNX_REQUIRE(plan->live_graphs.load() == 0, TNX_ERR_ARG,
+                  "the plan still has captured graphs (tnx_lanczos_graph_create); free them first");
+    delete plan;
+  });
+}
+
+static void validate_plan(const tnx_plan *plan, hipStream_t s) {
+  switch (plan->kind) {
+    case PlanKind::Pairwise: validate_pairwise(static_cast<const PairwisePlan &>(*plan), s); break;
+    case PlanKind::Matvec:
+    case PlanKind::EnvLeft:
+    case PlanKind::EnvRight: validate_fused(plan, s); break;
+    default: break;  // transpose plans run no GEMM
+  }
 }
 
 static GemmProgram *stage_program(tnx_plan *plan, int stage) {
@@ -1016,6 +1041,14 @@ int tnx_plan_set_tiling(tnx_plan *plan, int32_t stage, int32_t shape, int32_t kc
     c.xcd = xcd ? 1 : 0;
     set_tiling(*g, c, (hipStream_t)stream);
     ++plan->generation;
+    if (validation_enabled()) validate_plan(plan, (hipStream_t)stream);
+  });
+}
+
+int tnx_plan_validate(const tnx_plan *plan, void *stream) {
+  return guarded([&] {
+    TNX_REQUIRE(plan, TNX_ERR_ARG, "null plan");
+    validate_plan(plan, (hipStream_t)stream);
   });
 }
 

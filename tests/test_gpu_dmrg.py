@@ -301,3 +301,22 @@ def test_dmrg_rejects_mixed_and_dense(gpu):
         dense.add_node(i, DenseTensor(t.todense(), t.indices))
     with pytest.raises(TypeError, match="uniform tensor types"):
         dmrg(dense, mps, DMRGConfig(max_bond_dim=8, num_sweeps=1))
+
+
+def test_plan_with_live_graph_refuses_to_be_freed(gpu, state8):
+    """A captured graph launches its plan's work lists: tnx_plan_free refuses
+    the plan while the graph lives, and accepts it once the graph is freed."""
+    from tenax_amd import _lib
+
+    mpo, mps = state8
+    prob, _ = _problem(mpo, mps, 2)
+    lib = _lib.load()
+    g = prob.lanczos_graph(10, 1e-12, False, 0)
+    try:
+        assert lib.tnx_plan_free(prob.plan.ptr) == _lib.TNX_ERR_ARG
+        assert "captured graphs" in _lib.last_error()
+    finally:
+        g.close()
+    # the plan stays usable (it was not freed); its owner releases it later
+    e, n = prob.lanczos(10, 1e-12, False, 0)
+    assert n == 10 and np.isfinite(e)
