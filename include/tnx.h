@@ -142,6 +142,20 @@ int tnx_plan_get_tiling(const tnx_plan *plan, int32_t stage, int32_t *shape, int
 int tnx_contract_execute(const tnx_plan *plan, const double *a, const double *b,
                          double *c, int32_t compute, void *stream);
 
+/* Adjoint (vector-Jacobian product) of tnx_contract_execute: given g_c laid
+ * out like the plan's output, writes g_a = dL/dA laid out like A and
+ * g_b = dL/dB laid out like B (blocks no pair touches are zero); either may
+ * be null when not wanted.  The adjoint work lists are derived once from the
+ * forward plan's pair list and kept in the plan: dB_j = sum A_i^T G_o in one
+ * grouped launch (A read in the other layout), dA_i = sum G_o B_j^T in one
+ * grouped launch after a block-wise transpose of B (and, for a K-first A, a
+ * transpose of the result into place).  Replaces the reverse pass of
+ * jax.custom_vjp over the block contraction (the differentiated per-block
+ * einsums of contractor.py:444-476; the custom_vjp of ad_utils.py:694-764
+ * calls it through the CTM step).  The plan owns the scratch (transposed B). */
+int tnx_contract_vjp(const tnx_plan *plan, const double *g_c, const double *a, const double *b,
+                     double *g_a, double *g_b, int32_t compute, void *stream);
+
 /* Plain per-block GPU path: one GEMM launch per block pair (the structure of
  * contractor.py:444-476 on a GPU).  Same arithmetic; used as the comparison
  * baseline. */

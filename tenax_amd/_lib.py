@@ -87,6 +87,7 @@ SIGNATURES = {
     "tnx_plan_get_tiling": (C.c_int, [_p, _i32, _PI32, _PI32, _PI32, _PI32]),
     "tnx_contract_execute": (C.c_int, [_p, _p, _p, _p, _i32, _p]),
     "tnx_contract_execute_perblock": (C.c_int, [_p, _p, _p, _p, _i32, _p]),
+    "tnx_contract_vjp": (C.c_int, [_p, _p, _p, _p, _p, _p, _i32, _p]),
     "tnx_plan_transpose": (C.c_int, [C.POINTER(Blocks), C.POINTER(Leg), _PI32, _p, C.POINTER(_p)]),
     "tnx_transpose_execute": (C.c_int, [_p, _p, _p, _p]),
     "tnx_plan_validate": (C.c_int, [_p, _p]),

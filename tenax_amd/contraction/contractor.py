@@ -34,6 +34,7 @@ import torch
 from tenax_amd import _lib
 from tenax_amd.contraction import plans
 from tenax_amd.core.index import Label, TensorIndex
+from tenax_amd.core.symmetry import U1Symmetry
 from tenax_amd.core.tensor import DenseTensor, SymmetricTensor, Tensor, _transpose_raw
 
 _CHARS = string.ascii_lowercase + string.ascii_uppercase
@@ -115,9 +116,40 @@ def _labels_to_subscripts(tensors: Sequence[Tensor], output_labels: Sequence[Lab
 
 
 # ---------------------------------------------------------------- dense
+def _trivial_legs(indices):
+    """Legs of the same dimensions, flows and labels with every charge 0: a
+    dense tensor is then one block of the block engine."""
+    sym = U1Symmetry()
+    return tuple(TensorIndex(sym, np.zeros(ix.dim, np.int32), ix.flow, ix.label) for ix in indices)
+
+
+def _as_one_block(t: DenseTensor) -> SymmetricTensor:
+    idx = _trivial_legs(t.indices)
+    r = len(idx)
+    shape = np.array([[ix.dim for ix in idx]], np.int32).reshape(1, r)
+    return SymmetricTensor._wrap(idx, np.zeros((1, r), np.int32), shape, np.zeros(1, np.int64),
+                                 t.todense().reshape(-1))
+
+
 def _contract_dense(tensors, subscripts, output_indices, optimize="auto"):
-    arrays = [t.todense() for t in tensors]
-    return DenseTensor(torch.einsum(subscripts, *arrays), output_indices)
+    """contractor.py:150-200 on the block engine: every float64 operand is one
+    block with trivial charges, and the network is contracted pairwise by the
+    grouped GEMM (differentiable through its adjoint).  Operands of another
+    dtype, or with a trace inside one operand, go through torch.einsum
+    (rocBLAS)."""
+    lhs, so = subscripts.split("->")
+    subs = lhs.split(",")
+    dense_ok = all(t.todense().dtype == torch.float64 and t.todense().numel() > 0 for t in tensors) and all(
+        len(set(sb)) == len(sb) for sb in subs)
+    if not dense_ok:
+        arrays = [t.todense() for t in tensors]
+        return DenseTensor(torch.einsum(subscripts, *arrays), output_indices)
+    blocks = [_as_one_block(t) for t in tensors]
+    out = _contract_symmetric(blocks, subscripts, _trivial_legs(output_indices), optimize)
+    shape = tuple(ix.dim for ix in output_indices)
+    if out.n_blocks == 0:
+        return DenseTensor(torch.zeros(shape, dtype=torch.float64, device=out.data.device), output_indices)
+    return DenseTensor(out.data.reshape(shape), output_indices)
 
 
 # ---------------------------------------------------------------- symmetric
@@ -139,52 +171,39 @@ def _execute_pair(plan, A, B, out):
                   _lib.COMPUTE[_state["compute"]], _lib.stream_handle()))
 
 
-def _project_onto(R: SymmetricTensor, X: SymmetricTensor) -> torch.Tensor:
-    """R's blocks laid out in X's packed layout (zeros where R has no block)."""
-    out = torch.zeros(X.data.numel(), dtype=torch.float64, device=X.data.device)
-    kr = R.key_index()
-    xi, ri = [], []
-    for k, i in X.key_index().items():
-        j = kr.get(k)
-        if j is not None:
-            n = int(np.prod(X.shapes[i]))
-            xi.append(np.arange(X.offsets[i], X.offsets[i] + n, dtype=np.int64))
-            ri.append(np.arange(R.offsets[j], R.offsets[j] + n, dtype=np.int64))
-    if xi:
-        out[torch.from_numpy(np.concatenate(xi)).to(out.device)] = \
-            R.data[torch.from_numpy(np.concatenate(ri)).to(out.device)]
-    return out
-
-
 class _PairExecFn(torch.autograd.Function):
     """Grouped block GEMM C = A . B with its adjoint.
 
     Layouts: A = [free_a..., cont...] (or [cont..., free_a...]), B = [cont...,
-    free_b...], C = [free_a..., free_b...].  The adjoint is two more block
-    contractions on the same engine, dA = G . B^T and dB = A^T . G (sector
-    matching by charge, no conservation filter), each laid out in its
-    operand's own block table; blocks absent from an operand get no gradient.
+    free_b...], C = [free_a..., free_b...].  The backward is one library call,
+    tnx_contract_vjp on the forward plan: dA = G . B^T and dB = A^T . G, each
+    laid out in its operand's own block table, from work lists the library
+    derives once from the forward pair list; blocks no pair touches get zero.
     """
 
     @staticmethod
     def forward(ctx, a_data, b_data, plan, A, B, C, sa, sb, sc):
         if C.data.numel():
             _execute_pair(plan, A, B, C.data)
-        ctx.meta = (A, B, C, sa, sb, sc)
+        ctx.meta = (plan, A, B)
         return C.data
 
     @staticmethod
     def backward(ctx, g):
-        A, B, C, sa, sb, sc = ctx.meta
-        G = C._with_data(g.contiguous())
-        ga = gb = None
-        with torch.no_grad():
-            if ctx.needs_input_grad[0]:
-                R = contract_pair(G, B, sc, sb, sa, A.indices, _lib.MODE_MULTI_REACHABLE, None)
-                ga = _project_onto(R, A)
-            if ctx.needs_input_grad[1]:
-                R = contract_pair(A, G, sa, sc, sb, B.indices, _lib.MODE_MULTI_REACHABLE, None)
-                gb = _project_onto(R, B)
+        plan, A, B = ctx.meta
+        ga = torch.empty_like(A.data) if ctx.needs_input_grad[0] else None
+        gb = torch.empty_like(B.data) if ctx.needs_input_grad[1] else None
+        if ga is None and gb is None:
+            return None, None, None, None, None, None, None, None, None
+        g = g.contiguous()
+        lib = _lib.load()
+        _lib.check(lib.tnx_contract_vjp(plan.ptr, _lib.dptr(g), _lib.dptr(A.data), _lib.dptr(B.data),
+                                        _lib.dptr(ga) if ga is not None else None,
+                                        _lib.dptr(gb) if gb is not None else None,
+                                        _lib.COMPUTE[_state["compute"]], _lib.stream_handle()))
+        if WORK["enabled"]:
+            WORK["flops"] += 2.0 * plan.stats()["flops"]
+            WORK["launches"] += 2
         return ga, gb, None, None, None, None, None, None, None
 
 

@@ -19,6 +19,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <map>
+#include <mutex>
 #include <numeric>
 
 namespace tnx {
@@ -587,6 +589,78 @@ void set_tiling(GemmProgram &prog, const TileCfg &cfg, hipStream_t s) {
   prog.tiled = true;
 }
 
+// ------------------------------------------------------------------ block transposes
+// One block of a block-wise 2D transpose: in[r * cols + c] -> out[c * rows + r].
+struct BlockT {
+  int64_t in_off, out_off;
+  int32_t rows, cols;
+};
+
+// One workgroup per (block, 32 x 32 tile): the tile goes through LDS so that
+// both the reads and the writes are row-contiguous.
+__global__ void __launch_bounds__(256) block_transpose_kernel(const BlockT *__restrict__ list, int32_t n_blocks,
+                                                              const int64_t *__restrict__ tile_start,
+                                                              const double *__restrict__ in, double *__restrict__ out) {
+  __shared__ double t[32][33];
+  // find this workgroup's block (tile_start: exclusive prefix of tiles per block)
+  int lo = 0, hi = n_blocks - 1;
+  const int64_t w = blockIdx.x;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tile_start[mid] <= w) lo = mid; else hi = mid - 1;
+  }
+  const BlockT b = list[lo];
+  const int64_t tile = w - tile_start[lo];
+  const int tc = (b.cols + 31) / 32;
+  const int r0 = (int)(tile / tc) * 32, c0 = (int)(tile % tc) * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int y = ty; y < 32; y += 8) {
+    const int r = r0 + y, c = c0 + tx;
+    if (r < b.rows && c < b.cols) t[y][tx] = in[b.in_off + (int64_t)r * b.cols + c];
+  }
+  __syncthreads();
+  for (int y = ty; y < 32; y += 8) {
+    const int c = c0 + y, r = r0 + tx;
+    if (r < b.rows && c < b.cols) out[b.out_off + (int64_t)c * b.rows + r] = t[tx][y];
+  }
+}
+
+__global__ void fill_zero_kernel(int64_t n, double *__restrict__ p) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+static void zero_fill(double *p, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  const int64_t g = std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)g), dim3(256), 0, s, n, p);
+}
+
+// Block-wise transposes of a list (tile starts computed on the host).
+struct TransposeList {
+  DevBuf<BlockT> list;
+  DevBuf<int64_t> start;
+  int32_t n = 0;
+  int64_t tiles = 0;
+  void set(const std::vector<BlockT> &v, hipStream_t s) {
+    n = (int32_t)v.size();
+    std::vector<int64_t> st(v.size());
+    tiles = 0;
+    for (size_t i = 0; i < v.size(); ++i) {
+      st[i] = tiles;
+      tiles += (int64_t)((v[i].rows + 31) / 32) * ((v[i].cols + 31) / 32);
+    }
+    list.upload(v.data(), v.size(), s);
+    start.upload(st.data(), st.size(), s);
+  }
+  void run(const double *in, double *out, hipStream_t s) const {
+    if (!tiles) return;
+    TNX_REQUIRE(tiles < (int64_t(1) << 31), TNX_ERR_CAPACITY, "too many transpose tiles");
+    hipLaunchKernelGGL(block_transpose_kernel, dim3((unsigned)tiles), dim3(256), 0, s, list.p, n, start.p, in, out);
+    TNX_CHECK_HIP(hipGetLastError());
+  }
+};
+
 // ------------------------------------------------------------------ plan objects
 struct PairwisePlan : public tnx_plan_s {
   GemmProgram prog;
@@ -598,6 +672,13 @@ struct PairwisePlan : public tnx_plan_s {
   bool pb_ready = false;
   int64_t a_total = 0, b_total = 0;
   Extents ea, eb;  // operand block layouts (work-list validation)
+  // adjoint (tnx_contract_vjp), derived from the forward pair list on first use
+  std::mutex adj_mutex;
+  bool adj_ready = false;
+  GemmProgram adj_a, adj_b;   // dA = G . B^T, dB = A^T . G
+  TransposeList bt_list;      // B blocks transposed into bt (K x N -> N x K)
+  TransposeList at_list;      // KM forward: dA^T blocks (M x K) back to A's K x M
+  DevBuf<double> bt, at;      // transposed B blocks; dA^T staging (KM forward only)
 };
 
 // A pairwise plan's program against its operands' and its output's block layouts.
@@ -757,6 +838,95 @@ static void build_pairwise(PairwisePlan &P, const tnx_blocks *a, const tnx_block
   P.n_pairs = P.prog.n_pairs;
   P.n_tasks = P.prog.n_tasks;
   if (validation_enabled()) validate_pairwise(P, s);
+}
+
+// Upload a host-built program and let the device build its tile list.
+static void finalize_host_program(GemmProgram &g, int a_lay, const std::vector<GemmPair> &pairs,
+                                  const std::vector<GemmOut> &outs, hipStream_t s) {
+  g.a_lay = a_lay;
+  g.b_lay = LAY_KN;
+  g.n_pairs = (int32_t)pairs.size();
+  g.n_outs = (int32_t)outs.size();
+  if (!pairs.empty()) g.pairs.upload(pairs.data(), pairs.size(), s);
+  if (!outs.empty()) g.outs.upload(outs.data(), outs.size(), s);
+  finalize_program(g, s);
+}
+
+// Adjoint programs of a pairwise plan, from its forward pair list
+// (C_o = sum over pairs (i, j) -> o of A_i . B_j):
+//   dB_j = sum over its pairs A_i^T . G_o   (A read transposed: the other A layout)
+//   dA_i = sum over its pairs G_o . B_j^T   (B blocks transposed into bt first)
+// For a KM forward (A blocks stored K x M) dA_i^T is formed as M x K in a
+// staging buffer and transposed into place.  Output blocks that no pair
+// touches stay zero.
+static void ensure_adjoint(PairwisePlan &P, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(P.adj_mutex);
+  if (P.adj_ready) return;
+  const auto &ho = P.prog.h_outs;
+  const auto &hp = P.prog.h_pairs;
+  struct Use {
+    int32_t pair, out;
+  };
+  std::map<int64_t, std::vector<Use>> by_a, by_b;  // block offset -> its pairs, in forward order
+  for (int32_t o = 0; o < (int32_t)ho.size(); ++o)
+    for (int32_t k = ho[o].pair_begin; k < ho[o].pair_end; ++k)
+      if (hp[k].K > 0 && ho[o].M > 0 && ho[o].N > 0) {
+        by_a[hp[k].a_off].push_back(Use{k, o});
+        by_b[hp[k].b_off].push_back(Use{k, o});
+      }
+  const bool mk = P.prog.a_lay == LAY_MK;
+  // dB: outputs = B blocks (K x N, pitch ldb); A' = A^T, B' = G
+  std::vector<GemmPair> pb;
+  std::vector<GemmOut> ob;
+  std::vector<BlockT> bt;
+  for (const auto &kv : by_b) {
+    const GemmPair &f = hp[kv.second[0].pair];
+    const GemmOut &fo = ho[kv.second[0].out];
+    GemmOut g{kv.first, f.K, fo.N, f.ldb, (int32_t)pb.size(), 0, 0};
+    for (const Use &u : kv.second) {
+      const GemmPair &q = hp[u.pair];
+      const GemmOut &qo = ho[u.out];
+      TNX_REQUIRE(q.K == f.K && qo.N == fo.N && q.ldb == f.ldb, TNX_ERR_BAD_STRUCTURE,
+                  "pairs of one B block disagree on its shape");
+      pb.push_back(GemmPair{q.a_off, qo.c_off, qo.M, q.lda, qo.ldc, q.neg});
+    }
+    g.pair_end = (int32_t)pb.size();
+    ob.push_back(g);
+    bt.push_back(BlockT{kv.first, kv.first, f.K, f.ldb});  // K x N (pitch = N) -> N x K
+  }
+  // dA: outputs = A blocks (MK: M x K, pitch lda) or their transposes (KM)
+  std::vector<GemmPair> pa;
+  std::vector<GemmOut> oa;
+  std::vector<BlockT> at;
+  for (const auto &kv : by_a) {
+    const GemmPair &f = hp[kv.second[0].pair];
+    const GemmOut &fo = ho[kv.second[0].out];
+    TNX_REQUIRE(mk ? f.lda == f.K : f.lda == fo.M, TNX_ERR_BAD_STRUCTURE, "adjoint needs packed A blocks");
+    GemmOut g{kv.first, fo.M, f.K, f.K, (int32_t)pa.size(), 0, 0};
+    for (const Use &u : kv.second) {
+      const GemmPair &q = hp[u.pair];
+      const GemmOut &qo = ho[u.out];
+      TNX_REQUIRE(q.K == f.K && qo.M == fo.M, TNX_ERR_BAD_STRUCTURE, "pairs of one A block disagree on its shape");
+      // G_o (M x N_o, MK) . B_j^T (N_o x K, stored transposed in bt at the block's offset)
+      pa.push_back(GemmPair{qo.c_off, q.b_off, qo.N, qo.ldc, f.K, q.neg});
+    }
+    g.pair_end = (int32_t)pa.size();
+    oa.push_back(g);
+    if (!mk) at.push_back(BlockT{kv.first, kv.first, fo.M, f.K});  // M x K -> K x M
+  }
+  finalize_host_program(P.adj_b, mk ? LAY_KM : LAY_MK, pb, ob, s);
+  finalize_host_program(P.adj_a, LAY_MK, pa, oa, s);
+  P.bt_list.set(bt, s);
+  P.at_list.set(at, s);
+  P.bt.alloc(std::max<int64_t>(P.b_total, 1));
+  if (!mk) P.at.alloc(std::max<int64_t>(P.a_total, 1));
+  TNX_CHECK_HIP(hipStreamSynchronize(s));
+  if (validation_enabled()) {
+    const Extents ec = extents_of(P.out_shapes.data(), P.out_offsets.data(), P.out_nblocks, P.out_rank);
+    check_program(P.adj_b, P.ea, ec, P.eb, "adjoint dB", s);
+    check_program(P.adj_a, ec, P.eb, P.ea, "adjoint dA", s);
+  }
+  P.adj_ready = true;
 }
 
 static void ensure_perblock(PairwisePlan &P, hipStream_t s) {
@@ -1085,6 +1255,34 @@ int tnx_contract_execute_perblock(const tnx_plan *plan, const double *a, const d
     for (int k = 0; k < P.prog.n_pairs; ++k)
       launch_prepared(P.prog.a_lay, P.prog.b_lay, P.prog.signs, compute, P.pb_task_cnt[k], s, P.pb_tasks.p + P.pb_task_off[k],
                       P.pb_outs.p, P.prog.pairs.p, a, b, c, !P.pb_first[k]);
+  });
+}
+
+int tnx_contract_vjp(const tnx_plan *plan, const double *g_c, const double *a, const double *b, double *g_a,
+                     double *g_b, int32_t compute, void *stream) {
+  return guarded([&] {
+    TNX_REQUIRE(plan && plan->kind == PlanKind::Pairwise, TNX_ERR_ARG, "not a pairwise plan");
+    PairwisePlan &P = const_cast<PairwisePlan &>(static_cast<const PairwisePlan &>(*plan));
+    TNX_REQUIRE(compute >= 0 && compute <= 2, TNX_ERR_DTYPE, "unknown compute mode");
+    hipStream_t s = (hipStream_t)stream;
+    ensure_adjoint(P, s);
+    if (g_b) {
+      TNX_REQUIRE(g_c && a || P.adj_b.n_pairs == 0, TNX_ERR_ARG, "null buffer");
+      zero_fill(g_b, P.b_total, s);
+      launch_grouped_gemm(P.adj_b, a, g_c, g_b, compute, false, s);
+    }
+    if (g_a) {
+      TNX_REQUIRE(g_c && b || P.adj_a.n_pairs == 0, TNX_ERR_ARG, "null buffer");
+      zero_fill(g_a, P.a_total, s);
+      P.bt_list.run(b, P.bt.p, s);
+      if (P.prog.a_lay == LAY_MK) {
+        launch_grouped_gemm(P.adj_a, g_c, P.bt.p, g_a, compute, false, s);
+      } else {
+        launch_grouped_gemm(P.adj_a, g_c, P.bt.p, P.at.p, compute, false, s);
+        P.at_list.run(P.at.p, g_a, s);
+      }
+    }
+    TNX_CHECK_HIP(hipGetLastError());
   });
 }
 

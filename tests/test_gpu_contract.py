@@ -226,3 +226,48 @@ def test_zeros_and_random_normal_structure(gpu):
     # only through the device erfinv vs scipy's, within a few ulp
     assert max_rel_err(R, o) < 4 * np.finfo(float).eps
     assert float(Z.norm()) == 0.0
+
+
+@pytest.mark.parametrize("k_first", [False, True])
+@pytest.mark.parametrize("sym", [oc.U1(), oc.Zn(3)])
+def test_contract_vjp_matches_dense_chain_rule(gpu, k_first, sym):
+    """tnx_contract_vjp (through autograd of contract()): dL/dA and dL/dB of
+    <W, A . B> against the dense chain rule, for A with its contracted legs
+    last (M x K blocks) or first (K x M blocks), and only on stored blocks."""
+    from tenax_amd import contract
+
+    rng = np.random.default_rng(11 if k_first else 12)
+
+    def leg(lbl, flow, n):
+        ch = rng.integers(-2, 3, size=n) if sym.n is None else rng.integers(0, 3, size=n)
+        return oc.Leg(sym, ch, flow, lbl)
+
+    k1, k2 = leg("k", oc.OUT, 9), leg("j", oc.IN, 4)
+    a, p = leg("a", oc.IN, 7), leg("p", oc.OUT, 3)
+    A_legs = (k1, k2, a, p) if k_first else (a, p, k1, k2)
+    B_legs = (k1.flip_flow(), leg("q", oc.IN, 5), k2.flip_flow(), leg("b", oc.OUT, 6))
+    Ao, Bo = oc.SymT.random_normal(A_legs, (4, 1)), oc.SymT.random_normal(B_legs, (4, 2))
+    A, B = from_oracle(Ao), from_oracle(Bo)
+    a_leaf = A.data.detach().clone().requires_grad_(True)
+    b_leaf = B.data.detach().clone().requires_grad_(True)
+    Cp = contract(A._with_data(a_leaf), B._with_data(b_leaf))
+    W = torch.as_tensor(rng.standard_normal(Cp.data.numel())).cuda()
+    ga, gb = torch.autograd.grad(torch.dot(Cp.data, W), [a_leaf, b_leaf])
+    Wd = to_oracle(Cp._with_data(W)).todense()
+    Ad, Bd = Ao.todense(), Bo.todense()
+    la, lb, lc = "".join(l.label for l in A_legs), "".join(l.label for l in B_legs), "".join(Cp.labels())
+    ref_a = np.einsum(f"{lc},{lb}->{la}", Wd, Bd)
+    ref_b = np.einsum(f"{lc},{la}->{lb}", Wd, Ad)
+    got_a = to_oracle(A._with_data(ga)).todense()
+    got_b = to_oracle(B._with_data(gb)).todense()
+    # compare on stored blocks (entries outside every block have no gradient)
+    ma = np.zeros(Ad.shape, bool)
+    for k in Ao.blocks:
+        ma[np.ix_(*[np.nonzero(l.charges == q)[0] for l, q in zip(A_legs, k)])] = True
+    mb = np.zeros(Bd.shape, bool)
+    for k in Bo.blocks:
+        mb[np.ix_(*[np.nonzero(l.charges == q)[0] for l, q in zip(B_legs, k)])] = True
+    scale = max(1.0, np.abs(ref_a).max(), np.abs(ref_b).max())
+    assert np.abs((got_a - ref_a)[ma]).max() < 1e-12 * scale
+    assert np.abs((got_b - ref_b)[mb]).max() < 1e-12 * scale
+    assert np.all(got_a[~ma] == 0) and np.all(got_b[~mb] == 0)
