@@ -48,6 +48,11 @@ class HOTRGConfig:
 
 _VALID = ("alternating", "horizontal", "vertical")
 
+# Isometries from the leading max_bond_dim singular triplets of each sector
+# only (block subspace iteration, tenax_amd.linalg._leading_isometries) when
+# the cut is the bond dimension alone; False: the full Jordan-Wielandt eigh.
+LEADING = True
+
 # Optional wall-clock accounting of the step phases (device-synchronised at
 # phase boundaries while enabled; used by the benchmark).
 TIMERS: dict = {"enabled": False}
@@ -78,7 +83,7 @@ _V = dict(env=({"left": "L", "right": "R", "up": "down", "down": "up"}),
 def _isometries(Ms, roles, max_bond_dim, svd_trunc_err):
     if all(isinstance(M, SymmetricTensor) for M in Ms):
         res = truncated_svd_symmetric_many(Ms, roles["left"], roles["right"], "a", max_bond_dim, svd_trunc_err,
-                                           absorb="isometries")
+                                           absorb="isometries", leading=LEADING)
         return [(U, Vh) for U, _, Vh, _ in res]
     out = []
     for M in Ms:

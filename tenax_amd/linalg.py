@@ -289,6 +289,264 @@ def _factor_all(mats, absorb):
     return out
 
 
+# Leading singular triplets by block subspace iteration, for a cut that is a
+# fixed count k (HOTRG's bond dimension): sector matrices at least
+# LEADING_MIN_RATIO * k on their short side iterate a block of p = 2k vectors,
+#   U = orth(M V),  V R = orth(M^T U),  U^T M V = R^T = X diag(s) Y^T,
+# until the Ritz triplets (U X, s, V Y) of the leading k have residual
+# |M V Y - U X diag(s)| <= LEADING_TOL * s_0 per column (the other residual
+# M^T U X - V Y diag(s) vanishes by construction).  orth is shifted
+# Cholesky QR in three passes (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa,
+# "Shifted Cholesky QR for computing the QR factorization of ill-conditioned
+# matrices", SIAM J. Sci. Comput. 42 (2020)): batched Gram products, p x p
+# Cholesky factors and triangular inverses, orthonormal to working precision
+# for condition numbers up to ~1/eps; rocSOLVER's batched Householder QR of
+# the same blocks is 50-100x slower.  The small SVD of R^T comes from the
+# eigh of its 2p-wide Jordan-Wielandt matrix, so s has the absolute accuracy
+# eps * s_0 of a direct SVD and each (u, v) keeps the SVD's pairing.  The
+# leading k converge at rate s_2k / s_k per half step (Z2 Ising environments
+# at chi = 48, 1152-wide sectors: 2-11 iterations).  A group whose Cholesky
+# factorisation breaks down, or that does not converge within
+# LEADING_MAX_ITER, falls back to the full Jordan-Wielandt eigh.
+LEADING_TOL = 2e-14
+LEADING_MAX_ITER = 40
+LEADING_MIN_RATIO = 4
+LEADING_STATS: dict = {"calls": 0, "matrices": 0, "iterations": 0, "fallback": 0, "fb_orth": 0, "fb_svd": 0,
+                       "fb_conv": 0, "complete_eigh_fail": 0, "complete_chol_fail": 0}
+# Ritz vectors of singular values below this fraction of the largest are not
+# resolved by any FP64 factorisation (LAPACK's to eps * s_0 / gap); the
+# iteration requires convergence of the leading ones above it only.
+LEADING_RESOLVED = 1e-11
+
+
+_ORTH_TOL = 1e-12
+
+
+def _orth(X):
+    """Shifted Cholesky QR3 of a batch of tall blocks: (Q, R, ok), X = Q R."""
+    B, m, n = X.shape
+    eye = torch.eye(n, dtype=X.dtype, device=X.device).expand(B, n, n)
+    shift = 11.0 * (m * n + n * (n + 1)) * _EPS * (X * X).sum(dim=(1, 2))
+    ok = torch.ones(B, dtype=torch.bool, device=X.device)
+    Q, Rt = X, None
+    for k in range(3):
+        A = torch.bmm(Q.transpose(1, 2), Q)
+        if k == 0:
+            A = A + shift[:, None, None] * eye
+        L, info = torch.linalg.cholesky_ex(A)
+        ok &= info == 0
+        R = L.transpose(1, 2)
+        Q = torch.bmm(Q, torch.linalg.solve_triangular(R, eye, upper=True))
+        Rt = R if Rt is None else torch.bmm(R, Rt)
+    # beyond condition ~1/eps (numerically rank-deficient blocks) the passes
+    # can succeed without orthonormalising: such members are not ok
+    ok &= (torch.bmm(Q.transpose(1, 2), Q) - eye).abs().amax(dim=(1, 2)) <= _ORTH_TOL
+    return Q, Rt, ok
+
+
+# Eigenvalues of the Gram matrix of the shifted first pass (in [0, 1]) are
+# resolved to ~eps; directions below this (block singular values under
+# ~2e-12 |X|, i.e. sector singular values under the noise floor _NOISE) are
+# replaced by random directions in _orth_complete.
+_NULL_GRAM = 1e-14
+
+
+def _orth_complete(X, fill):
+    """Orthonormal basis of a batch of tall blocks that may be numerically
+    rank-deficient: (Q, ok) with span(Q) containing X's resolved column space
+    (columns of singular value above ~eps * |X|) and the rest completed from
+    ``fill`` (random blocks of X's shape).  Shifted Cholesky pass, eigh of
+    its Gram matrix with the resolved directions first, null directions
+    replaced, then two plain Cholesky QR passes, which keep the span of each
+    leading set of columns."""
+    B, m, n = X.shape
+    eye = torch.eye(n, dtype=X.dtype, device=X.device).expand(B, n, n)
+    shift = 11.0 * (m * n + n * (n + 1)) * _EPS * (X * X).sum(dim=(1, 2))
+    L1, info = torch.linalg.cholesky_ex(torch.bmm(X.transpose(1, 2), X) + shift[:, None, None] * eye)
+    ok = info == 0
+    Q1 = torch.bmm(X, torch.linalg.solve_triangular(L1.transpose(1, 2), eye, upper=True))
+    if not bool(ok.all()):
+        return Q1, ok
+    try:
+        lam, W = torch.linalg.eigh(torch.bmm(Q1.transpose(1, 2), Q1))
+    except torch.linalg.LinAlgError:
+        LEADING_STATS["complete_eigh_fail"] += 1
+        return Q1, torch.zeros_like(ok)
+    lam, W = lam.flip(-1), W.flip(-1)
+    good = lam > _NULL_GRAM
+    Q = torch.bmm(Q1, W) * torch.rsqrt(lam.clamp_min(_NULL_GRAM))[:, None, :]
+    Q = torch.where(good[:, None, :], Q, fill)
+    for _ in range(2):
+        L, info = torch.linalg.cholesky_ex(torch.bmm(Q.transpose(1, 2), Q))
+        ok &= info == 0
+        Q = torch.bmm(Q, torch.linalg.solve_triangular(L.transpose(1, 2), eye, upper=True))
+    if not bool(ok.all()):
+        LEADING_STATS["complete_chol_fail"] += 1
+    return Q, ok
+
+
+def _small_svd(T):
+    """SVD of a batch of square matrices from their Jordan-Wielandt eigh:
+    (X, s, Y) with T = X diag(s) Y^T, s descending."""
+    B, n, _ = T.shape
+    J = torch.zeros((B, 2 * n, 2 * n), dtype=T.dtype, device=T.device)
+    J[:, :n, n:] = T
+    J[:, n:, :n] = T.transpose(1, 2)
+    w, E = torch.linalg.eigh(J)
+    top = E[:, :, n:].flip(-1)
+    X = top[:, :n, :]
+    Y = top[:, n:, :]
+    X = X / torch.linalg.vector_norm(X, dim=1, keepdim=True).clamp_min(1e-300)
+    Y = Y / torch.linalg.vector_norm(Y, dim=1, keepdim=True).clamp_min(1e-300)
+    return X, w[:, n:].flip(-1).clamp_min(0.0), Y
+
+
+def _orthonormal_ordered(H, gen):
+    """Orthonormalise the columns of a batch of blocks in column order (each
+    leading set of columns keeps its span): two Cholesky QR passes, or, where
+    the columns are numerically dependent (Ritz vectors of singular values at
+    the noise level), Gram-Schmidt in two passes per column with dependent
+    columns replaced by random directions."""
+    B, n, k = H.shape
+    eye = torch.eye(k, dtype=H.dtype, device=H.device).expand(B, k, k)
+    Q = H
+    ok = True
+    for _ in range(2):
+        L, info = torch.linalg.cholesky_ex(torch.bmm(Q.transpose(1, 2), Q))
+        if bool((info != 0).any()):
+            ok = False
+            break
+        Q = torch.bmm(Q, torch.linalg.solve_triangular(L.transpose(1, 2), eye, upper=True))
+    if ok and float((torch.bmm(Q.transpose(1, 2), Q) - eye).abs().max()) <= _ORTH_TOL:
+        return Q
+    Q = torch.zeros_like(H)
+    for j in range(k):
+        v = H[:, :, j]
+        for attempt in range(2):
+            for _ in range(2):
+                if j:
+                    v = v - torch.bmm(Q[:, :, :j], torch.bmm(Q[:, :, :j].transpose(1, 2), v[:, :, None]))[:, :, 0]
+            nrm = torch.linalg.vector_norm(v, dim=1)
+            weak = nrm < 1e-6
+            if attempt == 1 or not bool(weak.any()):
+                break
+            v = torch.where(weak[:, None], torch.randn(v.shape, generator=gen, dtype=v.dtype, device=v.device), v)
+        Q[:, :, j] = v / nrm.clamp_min(1e-300)[:, None]
+    return Q
+
+
+def _leading_isometries(mats, k):
+    """(U, s, Vh) of the leading min(k, R, C) singular triplets of each sector
+    matrix, s descending; matrices too small for the iteration go through the
+    Jordan-Wielandt factorisation and are cut to the same count."""
+    out = [None] * len(mats)
+    groups: dict = {}
+    small = []
+    big = [j for j, M in enumerate(mats) if min(M.shape) >= LEADING_MIN_RATIO * k]
+    norms = {}
+    if big:
+        nrm = torch.stack([torch.linalg.matrix_norm(mats[j]) for j in big]).cpu().numpy()
+        norms = dict(zip(big, nrm))
+    for j, M in enumerate(mats):
+        # the iteration works on M / |M|_F; zero or non-finite matrices take
+        # the full factorisation
+        if j in norms and np.isfinite(norms[j]) and norms[j] > 0:
+            groups.setdefault(tuple(M.shape), []).append(j)
+        else:
+            small.append(j)
+    redo = list(small)
+    LEADING_STATS["calls"] += 1
+    for (R, C), js in groups.items():
+        dev = mats[js[0]].device
+        p = 2 * k
+        scale = torch.tensor([norms[j] for j in js], dtype=torch.float64, device=dev)
+        Mb = torch.stack([mats[j] for j in js]) / scale[:, None, None]
+        Mt = Mb.transpose(1, 2).contiguous()
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(0x51DE)
+        V, _, ok = _orth(torch.randn((len(js), C, p), generator=gen, dtype=torch.float64, device=dev))
+        Y = torch.bmm(Mb, V)
+        conv = torch.zeros(len(js), dtype=torch.bool, device=dev)
+        it = 0
+        complete = False  # rank-deficient members seen: complete null directions
+        svd_failed = False
+        while it < LEADING_MAX_ITER and js:
+            if complete:
+                U, ok1 = _orth_complete(Y, torch.randn(Y.shape, generator=gen, dtype=Y.dtype, device=dev))
+                Z = torch.bmm(Mt, U)
+                V, ok2 = _orth_complete(Z, torch.randn(Z.shape, generator=gen, dtype=Z.dtype, device=dev))
+                Rv = torch.bmm(V.transpose(1, 2), Z)  # = V^T M^T U, so R^T = U^T M V
+            else:
+                U, _, ok1 = _orth(Y)
+                V, Rv, ok2 = _orth(torch.bmm(Mt, U))
+            ok &= ok1 & ok2
+            ok_host = ok.cpu().numpy()
+            if not ok_host.all() and not complete:
+                # numerically rank-deficient blocks: redo this iteration with completion
+                complete = True
+                ok = torch.ones(len(js), dtype=torch.bool, device=dev)
+                continue
+            if not ok_host.all():
+                LEADING_STATS["fb_orth"] += int((~ok_host).sum())
+                # rank-deficient blocks leave for the fallback; the others
+                # restart this iteration from their current V
+                keep = torch.as_tensor(np.flatnonzero(ok_host), device=dev)
+                redo.extend(j for j, good in zip(js, ok_host) if not good)
+                js = [j for j, good in zip(js, ok_host) if good]
+                if not js:
+                    break
+                Mb, Mt, V = Mb[keep], Mt[keep], V[keep]
+                Y = torch.bmm(Mb, V)
+                ok = torch.ones(len(js), dtype=torch.bool, device=dev)
+                conv = torch.zeros(len(js), dtype=torch.bool, device=dev)
+                continue
+            it += 1
+            try:
+                X, s, Yv = _small_svd(Rv.transpose(1, 2).contiguous())
+            except torch.linalg.LinAlgError:
+                LEADING_STATS["fb_svd"] += len(js)
+                svd_failed = True
+                conv = torch.zeros_like(conv)
+                break
+            Y = torch.bmm(Mb, V)
+            Xk = X[:, :, :k]
+            Yk = Yv[:, :, :k].contiguous()
+            res = torch.bmm(Y, Yk) - torch.bmm(U, Xk) * s[:, None, :k]
+            r = torch.linalg.vector_norm(res, dim=1) / s[:, :1].clamp_min(1e-300)
+            resolved = s[:, :k] >= LEADING_RESOLVED * s[:, :1]
+            conv = ((r <= LEADING_TOL) | ~resolved).all(dim=1)
+            LEADING_STATS["iterations"] += 1
+            if bool(conv.all()):
+                break
+        LEADING_STATS["matrices"] += len(js)
+        if not js:
+            continue
+        conv_host = conv.cpu().numpy()
+        if not svd_failed:
+            LEADING_STATS["fb_conv"] += int((~conv_host).sum())
+        if not conv_host.any():
+            redo.extend(js)
+            continue
+        # the (x, y) halves of a Jordan-Wielandt eigenvector for s_i resolve
+        # to eps * s_0 / s_i: the kept ones are made orthonormal in s order
+        Xk = _orthonormal_ordered(Xk.contiguous(), gen)
+        Yk = _orthonormal_ordered(Yk, gen)
+        Uk = torch.bmm(U, Xk)
+        Vhk = torch.bmm(Yk.transpose(1, 2), V.transpose(1, 2))
+        sk = s[:, :k] * torch.tensor([norms[j] for j in js], dtype=torch.float64, device=dev)[:, None]
+        for b, j in enumerate(js):
+            if conv_host[b]:
+                out[j] = (Uk[b], sk[b], Vhk[b])
+            else:
+                redo.append(j)
+    if redo:
+        LEADING_STATS["fallback"] += len(redo) - len(small)
+        for j, (U, sv, Vh) in zip(redo, _factor_all([mats[i] for i in redo], "isometries")):
+            n = min(k, sv.numel())
+            out[j] = (U[:, :n], sv[:n], Vh[:n, :])
+    return out
+
+
 # singular values below this fraction of the largest are treated as noise
 _NOISE = 1e-12
 
@@ -364,10 +622,16 @@ def _assemble(t: SymmetricTensor, lax, rax):
 
 
 def truncated_svd_symmetric_many(tensors, left_labels, right_labels, new_bond_label, max_singular_values=None,
-                                 max_truncation_err=None, normalize=False, absorb=None, per_sector=False):
+                                 max_truncation_err=None, normalize=False, absorb=None, per_sector=False,
+                                 leading=False):
     """truncated_svd_symmetric over several tensors with the same labels, whose
     sector eigenproblems share one batched eigh (e.g. many HOTRG instances).
-    Each tensor is truncated on its own (global order within the tensor)."""
+    Each tensor is truncated on its own (global order within the tensor).
+
+    ``leading`` (with absorb="isometries", a max_singular_values cut and no
+    truncation-error rule): only the leading max_singular_values triplets of
+    each sector are computed (block subspace iteration); the returned full
+    spectrum then holds those values only."""
     prepared = []
     all_mats = []
     for t in tensors:
@@ -379,6 +643,8 @@ def truncated_svd_symmetric_many(tensors, left_labels, right_labels, new_bond_la
         all_mats.extend(mats)
     if per_sector:
         factors = _factor_per_sector(all_mats, absorb)
+    elif leading and absorb == "isometries" and max_singular_values is not None and max_truncation_err is None:
+        factors = _leading_isometries(all_mats, max(1, int(max_singular_values)))
     else:
         factors = _factor_all(all_mats, absorb)
     if absorb in ("left", "right") and REFINE > 0 and not per_sector:
