@@ -687,18 +687,36 @@ static void validate_pairwise(const PairwisePlan &P, hipStream_t s) {
   check_program(P.prog, P.ea, P.eb, ec, "pairwise plan", s);
 }
 
-struct TDesc {
-  int64_t in_off, out_off, size;
-  int32_t rank, neg;  // neg: the block's Koszul sign is -1 (graded transpose)
-  int32_t shape[MAX_RANK];
-  int64_t stride[MAX_RANK];  // input stride of output dim d
+// One block of a leg permutation, with the output dimensions merged where
+// they are also adjacent and in order in the input (size-1 dimensions
+// dropped), so the index arithmetic runs over as few dimensions as possible,
+// in 32 bits (a block holds fewer than 2^31 elements).
+//   mode 0 (gather): each thread moves 4 output elements 256 apart; when the
+//     last output dimension is also the input's innermost (s[rank-1] == 1)
+//     both sides are contiguous across the wave.
+//   mode 1 (tile): the input's innermost dimension g and the output's
+//     innermost dimension f are both at least PERM_TILE_MIN long: one
+//     workgroup moves a 32 x 32 (g, f) tile through LDS for one index of the
+//     other dimensions, reading along g and writing along f.
+struct PDesc {
+  int64_t in_off, out_off;
+  int64_t task0;       // first workgroup of this block
+  int32_t size, rank;  // elements; merged rank
+  int32_t neg;         // the block's Koszul sign is -1 (graded transpose)
+  int32_t mode, g;     // mode; tile mode: output dimension with input stride 1
+  int32_t tg, tf;      // tile mode: 32-wide tiles along g and along the last dimension
+  int32_t pad;
+  int32_t n[MAX_RANK];  // merged output dimensions
+  int32_t s[MAX_RANK];  // input stride of each
+  int32_t o[MAX_RANK];  // output stride of each
 };
+constexpr int PERM_EPT = 4;        // gather mode: elements per thread
+constexpr int PERM_TILE_MIN = 16;  // tile mode needs both tiled dimensions at least this long
 
 struct TransposePlan : public tnx_plan_s {
-  DevBuf<TDesc> desc;
-  DevBuf<int32_t> task_blk;
-  DevBuf<int64_t> task_start;
-  int32_t n_tasks_t = 0;
+  DevBuf<PDesc> desc;
+  int32_t n_desc = 0;
+  int64_t n_wg = 0;
 };
 
 static void build_pairwise(PairwisePlan &P, const tnx_blocks *a, const tnx_blocks *b, int a_kfirst, int b_kfirst,
@@ -975,20 +993,69 @@ __global__ void transpose_code_kernel(int nb, int rank, const int32_t *__restric
   ids[i] = (uint32_t)i;
 }
 
-__global__ void transpose_kernel(const TDesc *__restrict__ desc, const int32_t *__restrict__ tblk,
-                                 const int64_t *__restrict__ tstart, const double *__restrict__ in,
-                                 double *__restrict__ out) {
-  const TDesc d = desc[tblk[blockIdx.x]];
-  const int64_t e = tstart[blockIdx.x] + threadIdx.x;
-  if (e >= d.size) return;
-  int64_t r = e, src = 0;
-  for (int k = d.rank - 1; k >= 0; --k) {
-    const int64_t ix = r % d.shape[k];
-    r /= d.shape[k];
-    src += ix * d.stride[k];
+__global__ __launch_bounds__(256) void permute_kernel(const PDesc *__restrict__ desc, int32_t n_desc,
+                                                      const double *__restrict__ in, double *__restrict__ out) {
+  __shared__ double tile[32][33];
+  // this workgroup's block: the last with task0 <= blockIdx.x (wave-uniform search)
+  const int64_t w = blockIdx.x;
+  int lo = 0, hi = n_desc - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (desc[mid].task0 <= w) lo = mid; else hi = mid - 1;
   }
-  const double v = in[d.in_off + src];
-  out[d.out_off + e] = d.neg ? -v : v;
+  const PDesc d = desc[lo];
+  const int32_t t = (int32_t)(w - d.task0);
+  const double sg = d.neg ? -1.0 : 1.0;
+  const double *src = in + d.in_off;
+  double *dst = out + d.out_off;
+  if (d.mode == 0) {
+    int32_t e[PERM_EPT], off[PERM_EPT];
+#pragma unroll
+    for (int j = 0; j < PERM_EPT; ++j) {
+      e[j] = t * (256 * PERM_EPT) + (int32_t)threadIdx.x + 256 * j;
+      int32_t r = e[j] < d.size ? e[j] : 0, x = 0;
+      for (int k = d.rank - 1; k > 0; --k) {
+        const int32_t q = r / d.n[k];
+        x += (r - q * d.n[k]) * d.s[k];
+        r = q;
+      }
+      off[j] = x + r * d.s[0];
+    }
+    double v[PERM_EPT];
+#pragma unroll
+    for (int j = 0; j < PERM_EPT; ++j) v[j] = src[off[j]];  // in bounds: off of element 0 for the idle lanes
+#pragma unroll
+    for (int j = 0; j < PERM_EPT; ++j)
+      if (e[j] < d.size) dst[e[j]] = sg * v[j];
+    return;
+  }
+  const int f = d.rank - 1, g = d.g;
+  const int32_t per = d.tg * d.tf;
+  int32_t outer = t / per;
+  const int32_t rem = t - outer * per;
+  const int32_t g0 = (rem / d.tf) * 32, f0 = (rem % d.tf) * 32;
+  int32_t ib = 0, ob = 0;
+  for (int k = d.rank - 1; k >= 0; --k) {
+    if (k == g || k == f) continue;
+    const int32_t q = outer / d.n[k];
+    const int32_t ix = outer - q * d.n[k];
+    outer = q;
+    ib += ix * d.s[k];
+    ob += ix * d.o[k];
+  }
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int32_t ng = d.n[g], nf = d.n[f];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int32_t gg = g0 + tx, ff = f0 + ty + 8 * j;
+    if (gg < ng && ff < nf) tile[ty + 8 * j][tx] = src[ib + gg + ff * d.s[f]];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int32_t gg = g0 + ty + 8 * j, ff = f0 + tx;
+    if (gg < ng && ff < nf) dst[ob + gg * d.o[g] + ff] = sg * tile[tx][ty + 8 * j];
+  }
 }
 
 // Koszul sign of a block under a leg permutation (tensor.py:32-50, 853-856):
@@ -1039,14 +1106,11 @@ static void build_transpose(TransposePlan &P, const tnx_blocks *a, const tnx_leg
   P.out_keys.resize((size_t)nb * r);
   P.out_shapes.resize((size_t)nb * r);
   P.out_offsets.resize(nb);
-  std::vector<TDesc> desc(nb);
-  std::vector<int32_t> tblk;
-  std::vector<int64_t> tstart;
-  int64_t off = 0;
+  std::vector<PDesc> desc(nb);
+  int64_t off = 0, wg = 0;
   for (int k = 0; k < nb; ++k) {
     const int i = (int)h_order[k];
-    TDesc d{};
-    d.rank = r;
+    PDesc d{};
     d.neg = graded ? koszul_negative(a->keys + (int64_t)i * r, perm, r) : 0;
     d.in_off = a->offsets[i];
     d.out_off = off;
@@ -1056,28 +1120,61 @@ static void build_transpose(TransposePlan &P, const tnx_blocks *a, const tnx_leg
       st[l] = acc;
       acc *= a->shapes[(int64_t)i * r + l];
     }
+    TNX_REQUIRE(acc < (int64_t(1) << 31) - 4096, TNX_ERR_CAPACITY, "a block of 2^31 - 4096 or more elements");
+    // output dimensions with their input strides: size-1 ones dropped,
+    // neighbours merged where the input holds them adjacent and in order
+    int nr = 0;
     for (int dd = 0; dd < r; ++dd) {
       P.out_keys[(size_t)k * r + dd] = a->keys[(int64_t)i * r + perm[dd]];
-      P.out_shapes[(size_t)k * r + dd] = a->shapes[(int64_t)i * r + perm[dd]];
-      d.shape[dd] = a->shapes[(int64_t)i * r + perm[dd]];
-      d.stride[dd] = st[perm[dd]];
+      const int32_t n = a->shapes[(int64_t)i * r + perm[dd]];
+      P.out_shapes[(size_t)k * r + dd] = n;
+      if (n == 1) continue;
+      const int32_t sd = (int32_t)st[perm[dd]];
+      if (nr > 0 && (int64_t)d.s[nr - 1] == (int64_t)sd * n) {
+        d.n[nr - 1] *= n;
+        d.s[nr - 1] = sd;
+      } else {
+        d.n[nr] = n;
+        d.s[nr] = sd;
+        ++nr;
+      }
     }
-    d.size = acc;
+    if (nr == 0) {
+      d.n[0] = 1;
+      d.s[0] = 1;
+      nr = 1;
+    }
+    d.rank = nr;
+    d.o[nr - 1] = 1;
+    for (int l = nr - 2; l >= 0; --l) d.o[l] = d.o[l + 1] * d.n[l + 1];
+    d.size = (int32_t)acc;
+    d.mode = 0;
+    int64_t tasks = (acc + 256 * PERM_EPT - 1) / (256 * PERM_EPT);
+    if (acc > 0 && d.s[nr - 1] != 1) {
+      int gdim = -1;
+      for (int l = 0; l < nr; ++l)
+        if (d.s[l] == 1) gdim = l;
+      if (gdim >= 0 && d.n[gdim] >= PERM_TILE_MIN && d.n[nr - 1] >= PERM_TILE_MIN) {
+        d.mode = 1;
+        d.g = gdim;
+        d.tg = (d.n[gdim] + 31) / 32;
+        d.tf = (d.n[nr - 1] + 31) / 32;
+        tasks = (acc / ((int64_t)d.n[gdim] * d.n[nr - 1])) * d.tg * d.tf;
+      }
+    }
+    d.task0 = wg;
+    wg += tasks;
     P.out_offsets[k] = off;
     desc[k] = d;
-    for (int64_t e = 0; e < acc; e += 256) {
-      tblk.push_back(k);
-      tstart.push_back(e);
-    }
     off += acc;
   }
+  TNX_REQUIRE(wg < (int64_t(1) << 31), TNX_ERR_CAPACITY, "too many permutation workgroups");
   P.out_total = off;
   P.bytes = 16.0 * off;
   P.desc.upload(desc.data(), nb, s);
-  P.task_blk.upload(tblk.data(), tblk.size(), s);
-  P.task_start.upload(tstart.data(), tstart.size(), s);
-  P.n_tasks_t = (int32_t)tblk.size();
-  P.n_tasks = P.n_tasks_t;
+  P.n_desc = nb;
+  P.n_wg = wg;
+  P.n_tasks = (int32_t)wg;
   TNX_CHECK_HIP(hipStreamSynchronize(s));
 }
 
@@ -1310,10 +1407,10 @@ int tnx_transpose_execute(const tnx_plan *plan, const double *a, double *out, vo
   return guarded([&] {
     TNX_REQUIRE(plan && plan->kind == PlanKind::Transpose, TNX_ERR_ARG, "not a transpose plan");
     const TransposePlan &P = static_cast<const TransposePlan &>(*plan);
-    if (P.n_tasks_t == 0) return;
+    if (P.n_wg == 0) return;
     TNX_REQUIRE(a && out, TNX_ERR_ARG, "null buffer");
-    hipLaunchKernelGGL(transpose_kernel, dim3(P.n_tasks_t), dim3(256), 0, (hipStream_t)stream, P.desc.p,
-                       P.task_blk.p, P.task_start.p, a, out);
+    hipLaunchKernelGGL(permute_kernel, dim3((unsigned)P.n_wg), dim3(256), 0, (hipStream_t)stream, P.desc.p, P.n_desc,
+                       a, out);
     TNX_CHECK_HIP(hipGetLastError());
   });
 }
