@@ -10,15 +10,17 @@ matvec (the reference's loop structure) on the same problem, and one full
 left-to-right DMRG sweep over the chi = 2048 chain (phases timed, two centre
 updates also on the per-block path).
 
-With ``--workload hotrg`` it instead runs the batched Z2 HOTRG workload: 64
-2D-Ising temperatures T = 2.0 + 0.6 k/63, chi = 48, 30 coarse-graining steps,
-all instances in lockstep (one batched eigh per step).  With ``--workload ctm``
-the iPEPS CTMRG workload: Heisenberg gate, D = 4, d = 2, chi = 64, QR
+The same line carries two summary keys: "c5", the batched Z2 HOTRG workload
+(64 2D-Ising temperatures T = 2.0 + 0.6 k/63, chi = 48, 30 coarse-graining
+steps, all instances in lockstep, one batched factorisation per step), and
+"c4", the iPEPS CTMRG workload (Heisenberg gate, D = 4, d = 2, chi = 64, QR
 projectors, site tensor N(0, 1) from seed 3: CTM sweeps on the dense and the
-block-sparse path, the 100-sweep AD forward and its implicit backward.
+block-sparse path, the 100-sweep AD forward and one GMRES cycle of its
+implicit backward).  ``--workload hotrg|ctm`` runs either alone as the line's
+headline (with its CPU baseline).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-sweep]
-                       [--workload dmrg|hotrg]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-sweep] [--no-c4c5]
+                       [--workload dmrg|hotrg|ctm]
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
 
@@ -434,7 +436,7 @@ def run_hotrg(args):
     }
     if not args.no_cpu:
         result["cpu_baseline"] = hotrg_cpu_baseline(chi)
-    print(json.dumps(result), flush=True)
+    return result
 
 
 def hotrg_cpu_baseline(chi):
@@ -563,7 +565,7 @@ def run_ctm(args):
                                   "sample": f"one CTM sweep at chi={chi} by the oracle (NumPy FP64 einsum / LAPACK, "
                                             "OpenBLAS threads = cores); value = block-path flops / CPU time"}
         log(f"ctm cpu sweep {cpu:.2f}s")
-    print(json.dumps(result), flush=True)
+    return result
 
 
 def main():
@@ -584,6 +586,8 @@ def main():
                     help="also time one full sweep from the C3 state on both paths")
     ap.add_argument("--no-reorth", action="store_true", help="C3 with the reference's plain Lanczos")
     ap.add_argument("--workload", choices=("dmrg", "hotrg", "ctm"), default="dmrg")
+    ap.add_argument("--no-c4c5", dest="c4c5", action="store_false",
+                    help="leave the HOTRG (c5) and CTMRG (c4) summary keys out of the DMRG line")
     ap.add_argument("--matvec-only", action="store_true", help="fused matvec and stage timing only (profiling)")
     ap.add_argument("--calibrate-copy", action="store_true", help="run a 512 MiB device copy (counter calibration)")
     ap.add_argument("--pin-tiling", action="append", default=[], metavar="STAGE:SHAPE,KCHUNK,XCD",
@@ -600,10 +604,10 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.workload == "hotrg":
-        run_hotrg(args)
+        print(json.dumps(run_hotrg(args)), flush=True)
         return
     if args.workload == "ctm":
-        run_ctm(args)
+        print(json.dumps(run_ctm(args)), flush=True)
         return
     import torch
 
@@ -775,7 +779,29 @@ def main():
     if not args.no_parity:
         result["parity"] = run_c1_parity()
         log(f"parity {result['parity']}")
+    if args.c4c5:
+        result["c5"] = c5_summary(args)
+        result["c4"] = c4_summary(args)
     print(json.dumps(result), flush=True)
+
+
+def c5_summary(args):
+    """The batched HOTRG workload (64 Z2 Ising temperatures, chi=48, 30 steps)
+    as a key of the default line: time, block-contraction GFLOP/s, phase split
+    and the worst free-energy error against Onsager."""
+    r = run_hotrg(argparse.Namespace(**{**vars(args), "no_cpu": True}))
+    out = {k: r[k] for k in ("metric", "value", "unit", "ms_per_step", "config", "seconds", "contraction_flops",
+                             "launches", "phases_s_first8")}
+    out["max_rel_err_vs_onsager"] = r["free_energy"]["max_rel_err_vs_onsager"]
+    return out
+
+
+def c4_summary(args):
+    """The iPEPS CTMRG workload (D=4, chi=64, QR projectors) as a key of the
+    default line: sweep time on the block and dense paths, AD forward and one
+    GMRES cycle of the implicit backward."""
+    r = run_ctm(argparse.Namespace(**{**vars(args), "no_cpu": True}))
+    return {k: r[k] for k in ("metric", "value", "unit", "ms_per_step", "config", "sweep", "ad")}
 
 
 if __name__ == "__main__":
