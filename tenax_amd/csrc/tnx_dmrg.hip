@@ -1158,6 +1158,10 @@ static void run_fused_perblock(FusedPlan &P, const double *s1A, const double *s1
 // reduction workgroups: enough loads in flight to stream a vector from HBM
 // (256 left the dot products latency-bound at ~1.5 TB/s)
 constexpr int LZ_G = 1024;
+// Elements of w per thread held in registers by the fused re-orthogonalising
+// kernels (n up to LZ_FUSED_RMAX * LZ_G * 256 = 6.3M doubles); larger
+// problems take the unfused kernels.
+constexpr int LZ_FUSED_RMAX = 24;
 
 struct LzState {
   int32_t stopped;
@@ -1349,6 +1353,118 @@ __global__ void lz_axpy_norm_kernel(double *__restrict__ w, const double *__rest
   for (; i < n; i += step) {
     double t = w[i] - v[i] * a;
     if (k > 0) t = t - vp[i] * b;
+    w[i] = t;
+    acc += t * t;
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// Re-orthogonalising step, first half: the three-term update of w (as
+// lz_axpy_kernel, with alpha[k] summed from the <v, w> partials `apart` by
+// every workgroup as in lz_axpy_norm_kernel) fused with the per-workgroup
+// partial sums of <V[j], w> for j = 0..k (the quantities of
+// dot_multi_partial_kernel; the same element assignment, the workgroup's sum
+// reduced in another fixed order).  A thread keeps its R <= RMAX updated
+// elements of w in registers, so w is read once instead of k + 2 times.
+// Launched with LZ_G workgroups; mpart (the k + 1 rows of partials) must not
+// overlap apart.
+template <int RMAX>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void lz_axpy_multidot_kernel(double *__restrict__ w, const double *__restrict__ V,
+                                                               int64_t n, int k, const double *__restrict__ apart,
+                                                               int G, double *__restrict__ alpha,
+                                                               const double *__restrict__ beta,
+                                                               double *__restrict__ mpart,
+                                                               const LzState *__restrict__ st) {
+  if (st->stopped) return;
+  __shared__ double sh[16];
+  const double a = block_sum(apart, G);
+  if (blockIdx.x == 0 && threadIdx.x == 0) alpha[k] = a;
+  const double b = k > 0 ? beta[k] : 0.0;
+  const double *v = V + (int64_t)k * n;
+  const double *vp = k > 0 ? V + (int64_t)(k - 1) * n : v;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  const int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  // byte offsets of the thread's elements inside one basis vector (32-bit:
+  // n * 8 < 2^32 is checked by the launcher); the basis vectors are read
+  // through buffer resources spanning exactly one vector, so offsets at or
+  // past its end read 0 and need no test
+  uint32_t off[RMAX];
+  double wr[RMAX];
+#pragma unroll
+  for (int r = 0; r < RMAX; ++r) {
+    const int64_t i = i0 + r * step;
+    off[r] = (uint32_t)(8 * i);
+    wr[r] = 0.0;
+    if (i < n) {
+      double t = w[i] - v[i] * a;
+      if (k > 0) t = t - vp[i] * b;
+      w[i] = t;
+      wr[r] = t;
+    }
+  }
+  const int32_t nbytes = (int32_t)(8 * n);
+  // Two basis vectors per pass (twice the loads in flight).  Each sum is
+  // reduced over the wave by butterfly shuffles and over the four waves
+  // through LDS in wave order: one barrier per pass.  Double-buffered LDS, so
+  // a pass's writes cannot meet the previous pass's reads.
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll 1
+  for (int j = 0, pass = 0; j <= k; j += 2, ++pass) {
+    const bool two = j + 1 <= k;
+    const double *x0 = V + (int64_t)j * n;
+    const __amdgpu_buffer_rsrc_t r0 = __builtin_amdgcn_make_buffer_rsrc((void *)x0, (short)0, nbytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t r1 =
+        __builtin_amdgcn_make_buffer_rsrc((void *)(two ? x0 + n : x0), (short)0, nbytes, 0x00020000);
+    double xa[RMAX], xb[RMAX];
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r) {
+      xa[r] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r0, (int32_t)off[r], 0, 0));
+      xb[r] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r1, (int32_t)off[r], 0, 0));
+    }
+    double acc0 = 0.0, acc1 = 0.0;
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r) {
+      acc0 += xa[r] * wr[r];
+      acc1 += xb[r] * wr[r];
+    }
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) {
+      acc0 += __shfl_xor(acc0, m, 64);
+      acc1 += __shfl_xor(acc1, m, 64);
+    }
+    double *buf = sh + (pass & 1) * 8;
+    if (lane == 0) {
+      buf[wave] = acc0;
+      buf[4 + wave] = acc1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      mpart[(int64_t)j * gridDim.x + blockIdx.x] = ((buf[0] + buf[1]) + buf[2]) + buf[3];
+      if (two) mpart[(int64_t)(j + 1) * gridDim.x + blockIdx.x] = ((buf[4] + buf[5]) + buf[6]) + buf[7];
+    }
+  }
+}
+
+// Re-orthogonalising step, second half: w = w - sum_j V[j]*c[j] (as
+// lz_reorth_kernel) fused with the partial sums of w.w (as
+// dot_partial_kernel over LZ_G workgroups: the same partials).
+__global__ __launch_bounds__(256) void lz_reorth_norm_kernel(double *__restrict__ w, const double *__restrict__ V,
+                                                             int64_t n, int nv, const double *__restrict__ c,
+                                                             double *__restrict__ part,
+                                                             const LzState *__restrict__ st) {
+  if (st->stopped) return;
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double t = w[i];
+#pragma unroll 8
+    for (int j = 0; j < nv; ++j) t = t - V[(int64_t)j * n + i] * c[j];
     w[i] = t;
     acc += t * t;
   }
@@ -1607,12 +1723,36 @@ static void lanczos_enqueue(const FusedPlan &P, const double *L, const double *W
     hipLaunchKernelGGL(dot_partial_kernel, rg, vb, 0, s, v, w, n, part, W.st);
     double *npart = part;  // partials of w.w
     if (reorth) {
-      hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), vb, 0, s, part, LZ_G, 1, k, tol, alpha, beta, W.st);
-      hipLaunchKernelGGL(lz_axpy_kernel, vg, vb, 0, s, w, v, vp, n, k, alpha, beta, W.st);
-      hipLaunchKernelGGL(dot_multi_partial_kernel, dim3(LZ_G, k + 1), vb, 0, s, V, n, k + 1, w, part, W.st);
-      hipLaunchKernelGGL(lz_multi_finalize_kernel, dim3(k + 1), vb, 0, s, part, LZ_G, rc, W.st);
-      hipLaunchKernelGGL(lz_reorth_kernel, vg, vb, 0, s, w, V, n, k + 1, rc, W.st);
-      hipLaunchKernelGGL(dot_partial_kernel, rg, vb, 0, s, w, w, n, part, W.st);
+      // classical Gram-Schmidt against V[0..k] after the three-term update;
+      // up to LZ_FUSED_RMAX elements per thread: two fused passes over V
+      const int64_t per_thread = (n + (int64_t)LZ_G * 256 - 1) / ((int64_t)LZ_G * 256);
+      double *mpart = part + LZ_G;  // the <v, w> partials are still being read
+      if (per_thread <= LZ_FUSED_RMAX && 8 * n < (int64_t(1) << 31)) {
+        auto multidot = [&](auto kern) {
+          hipLaunchKernelGGL(kern, rg, vb, 0, s, w, (const double *)V, n, k, (const double *)part, LZ_G, alpha,
+                             (const double *)beta, mpart, (const LzState *)W.st);
+        };
+        if (per_thread <= 4)
+          multidot(lz_axpy_multidot_kernel<4>);
+        else if (per_thread <= 8)
+          multidot(lz_axpy_multidot_kernel<8>);
+        else if (per_thread <= 12)
+          multidot(lz_axpy_multidot_kernel<12>);
+        else if (per_thread <= 16)
+          multidot(lz_axpy_multidot_kernel<16>);
+        else
+          multidot(lz_axpy_multidot_kernel<LZ_FUSED_RMAX>);
+        hipLaunchKernelGGL(lz_multi_finalize_kernel, dim3(k + 1), vb, 0, s, mpart, LZ_G, rc, W.st);
+        hipLaunchKernelGGL(lz_reorth_norm_kernel, rg, vb, 0, s, w, (const double *)V, n, k + 1,
+                           (const double *)rc, part, (const LzState *)W.st);
+      } else {
+        hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), vb, 0, s, part, LZ_G, 1, k, tol, alpha, beta, W.st);
+        hipLaunchKernelGGL(lz_axpy_kernel, vg, vb, 0, s, w, v, vp, n, k, alpha, beta, W.st);
+        hipLaunchKernelGGL(dot_multi_partial_kernel, dim3(LZ_G, k + 1), vb, 0, s, V, n, k + 1, w, part, W.st);
+        hipLaunchKernelGGL(lz_multi_finalize_kernel, dim3(k + 1), vb, 0, s, part, LZ_G, rc, W.st);
+        hipLaunchKernelGGL(lz_reorth_kernel, vg, vb, 0, s, w, V, n, k + 1, rc, W.st);
+        hipLaunchKernelGGL(dot_partial_kernel, rg, vb, 0, s, w, w, n, part, W.st);
+      }
     } else {
       npart = part + LZ_G;  // the <v, w> partials are still being read
       hipLaunchKernelGGL(lz_axpy_norm_kernel, rg, vb, 0, s, w, v, vp, n, k, (const double *)part, LZ_G, alpha,
