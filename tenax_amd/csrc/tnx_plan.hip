@@ -691,27 +691,38 @@ static void validate_pairwise(const PairwisePlan &P, hipStream_t s) {
 // they are also adjacent and in order in the input (size-1 dimensions
 // dropped), so the index arithmetic runs over as few dimensions as possible,
 // in 32 bits (a block holds fewer than 2^31 elements).
-//   mode 0 (gather): each thread moves 4 output elements 256 apart; when the
-//     last output dimension is also the input's innermost (s[rank-1] == 1)
-//     both sides are contiguous across the wave.
-//   mode 1 (tile): the input's innermost dimension g and the output's
-//     innermost dimension f are both at least PERM_TILE_MIN long: one
-//     workgroup moves a 32 x 32 (g, f) tile through LDS for one index of the
-//     other dimensions, reading along g and writing along f.
+//   mode 0 (gather): each thread moves 4 output elements 256 apart; used when
+//     the last output dimension is also the input's innermost (s[rank-1] ==
+//     1) and at least PERM_RUN long, so both sides move long contiguous runs.
+//   mode 1 (tile): one workgroup moves a tile of up to PERM_LDS elements
+//     through LDS.  The tile spans the input's innermost dimensions (whole,
+//     the outermost of them possibly in part) and the output's innermost
+//     dimensions, chosen so that the tile is read in contiguous input runs
+//     and written in contiguous output runs of at least PERM_RUN_MIN
+//     elements where the block allows; at most PERM_MAXT dimensions are
+//     tiled.  Short dimensions (e.g. the 24-wide sectors of a chi = 48 Z2
+//     tensor) are combined until the runs are long.
+constexpr int PERM_MAXT = 4;
+constexpr int PERM_LDS = 4096;  // doubles per tile (32 KB)
 struct PDesc {
   int64_t in_off, out_off;
   int64_t task0;       // first workgroup of this block
   int32_t size, rank;  // elements; merged rank
   int32_t neg;         // the block's Koszul sign is -1 (graded transpose)
-  int32_t mode, g;     // mode; tile mode: output dimension with input stride 1
-  int32_t tg, tf;      // tile mode: 32-wide tiles along g and along the last dimension
+  int32_t mode;
+  int32_t ptile;       // tile mode: elements of a full tile
   int32_t pad;
   int32_t n[MAX_RANK];  // merged output dimensions
   int32_t s[MAX_RANK];  // input stride of each
   int32_t o[MAX_RANK];  // output stride of each
+  int32_t t[MAX_RANK];  // tile mode: tile extent of each (1: not tiled)
+  int32_t c[MAX_RANK];  // tile mode: tiles along each
+  int32_t rd[PERM_MAXT], wd[PERM_MAXT];  // tile mode: tiled dimensions in input / output order, innermost first (-1: none)
+  int32_t lt[PERM_MAXT];                 // tile mode: LDS stride of wd[j] (tile stored in output order)
 };
-constexpr int PERM_EPT = 4;        // gather mode: elements per thread
-constexpr int PERM_TILE_MIN = 16;  // tile mode needs both tiled dimensions at least this long
+constexpr int PERM_EPT = 4;       // gather mode: elements per thread
+constexpr int PERM_RUN = 128;     // gather mode needs a contiguous last dimension at least this long
+constexpr int PERM_RUN_MIN = 16;  // tile mode: shortest run target tried
 
 struct TransposePlan : public tnx_plan_s {
   DevBuf<PDesc> desc;
@@ -995,7 +1006,7 @@ __global__ void transpose_code_kernel(int nb, int rank, const int32_t *__restric
 
 __global__ __launch_bounds__(256) void permute_kernel(const PDesc *__restrict__ desc, int32_t n_desc,
                                                       const double *__restrict__ in, double *__restrict__ out) {
-  __shared__ double tile[32][33];
+  __shared__ double lds[PERM_LDS];
   // this workgroup's block: the last with task0 <= blockIdx.x (wave-uniform search)
   const int64_t w = blockIdx.x;
   int lo = 0, hi = n_desc - 1;
@@ -1003,18 +1014,20 @@ __global__ __launch_bounds__(256) void permute_kernel(const PDesc *__restrict__ 
     const int mid = (lo + hi + 1) >> 1;
     if (desc[mid].task0 <= w) lo = mid; else hi = mid - 1;
   }
-  const PDesc d = desc[lo];
+  const PDesc &d = desc[lo];
   const int32_t t = (int32_t)(w - d.task0);
   const double sg = d.neg ? -1.0 : 1.0;
   const double *src = in + d.in_off;
   double *dst = out + d.out_off;
+  const int rank = d.rank;
   if (d.mode == 0) {
+    const int32_t size = d.size;
     int32_t e[PERM_EPT], off[PERM_EPT];
 #pragma unroll
     for (int j = 0; j < PERM_EPT; ++j) {
       e[j] = t * (256 * PERM_EPT) + (int32_t)threadIdx.x + 256 * j;
-      int32_t r = e[j] < d.size ? e[j] : 0, x = 0;
-      for (int k = d.rank - 1; k > 0; --k) {
+      int32_t r = e[j] < size ? e[j] : 0, x = 0;
+      for (int k = rank - 1; k > 0; --k) {
         const int32_t q = r / d.n[k];
         x += (r - q * d.n[k]) * d.s[k];
         r = q;
@@ -1026,35 +1039,86 @@ __global__ __launch_bounds__(256) void permute_kernel(const PDesc *__restrict__ 
     for (int j = 0; j < PERM_EPT; ++j) v[j] = src[off[j]];  // in bounds: off of element 0 for the idle lanes
 #pragma unroll
     for (int j = 0; j < PERM_EPT; ++j)
-      if (e[j] < d.size) dst[e[j]] = sg * v[j];
+      if (e[j] < size) dst[e[j]] = sg * v[j];
     return;
   }
-  const int f = d.rank - 1, g = d.g;
-  const int32_t per = d.tg * d.tf;
-  int32_t outer = t / per;
-  const int32_t rem = t - outer * per;
-  const int32_t g0 = (rem / d.tf) * 32, f0 = (rem % d.tf) * 32;
-  int32_t ib = 0, ob = 0;
-  for (int k = d.rank - 1; k >= 0; --k) {
-    if (k == g || k == f) continue;
-    const int32_t q = outer / d.n[k];
-    const int32_t ix = outer - q * d.n[k];
-    outer = q;
-    ib += ix * d.s[k];
-    ob += ix * d.o[k];
-  }
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  const int32_t ng = d.n[g], nf = d.n[f];
+  // tile mode: the tile's origin from the workgroup's tile coordinates, and
+  // the extent of each tiled dimension (edge tiles are cut at the block's end)
+  int32_t rT[PERM_MAXT], rS[PERM_MAXT], rL[PERM_MAXT], rE[PERM_MAXT];
+  int32_t wT[PERM_MAXT], wO[PERM_MAXT], wL[PERM_MAXT], wE[PERM_MAXT];
+  int32_t rdim[PERM_MAXT], wdim[PERM_MAXT];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int32_t gg = g0 + tx, ff = f0 + ty + 8 * j;
-    if (gg < ng && ff < nf) tile[ty + 8 * j][tx] = src[ib + gg + ff * d.s[f]];
+  for (int j = 0; j < PERM_MAXT; ++j) {
+    rdim[j] = d.rd[j];
+    wdim[j] = d.wd[j];
+    rT[j] = wT[j] = 1;
+    rS[j] = wO[j] = rL[j] = wL[j] = 0;
+    rE[j] = wE[j] = 1;
+  }
+  int32_t rem = t, ib = 0, ob = 0;
+  for (int k = rank - 1; k >= 0; --k) {
+    const int32_t ck = d.c[k], tk = d.t[k];
+    const int32_t q = rem / ck;
+    const int32_t x0 = (rem - q * ck) * tk;
+    rem = q;
+    ib += x0 * d.s[k];
+    ob += x0 * d.o[k];
+    const int32_t ext = min(tk, d.n[k] - x0);
+#pragma unroll
+    for (int j = 0; j < PERM_MAXT; ++j) {
+      if (rdim[j] == k) {
+        rT[j] = tk;
+        rS[j] = d.s[k];
+        rE[j] = ext;
+      }
+      if (wdim[j] == k) {
+        wT[j] = tk;
+        wO[j] = d.o[k];
+        wE[j] = ext;
+        wL[j] = d.lt[j];
+      }
+    }
+  }
+  // LDS stride of each read-order dimension: the tile is stored in output order
+#pragma unroll
+  for (int j = 0; j < PERM_MAXT; ++j)
+#pragma unroll
+    for (int i = 0; i < PERM_MAXT; ++i)
+      if (rdim[j] >= 0 && wdim[i] == rdim[j]) rL[j] = d.lt[i];
+  const int32_t P = d.ptile;
+  // read in input order (consecutive threads along the input's innermost
+  // dimensions), store in output order
+#pragma unroll 4
+  for (int32_t q = threadIdx.x; q < P; q += 256) {
+    int32_t rr = q, off = ib, li = 0;
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < PERM_MAXT; ++j) {
+      const int32_t qq = rr / rT[j];
+      const int32_t l = rr - qq * rT[j];
+      rr = qq;
+      ok = ok && l < rE[j];
+      off += l * rS[j];
+      li += l * rL[j];
+    }
+    if (ok) lds[li] = src[off];
   }
   __syncthreads();
+  // write in output order
+#pragma unroll 4
+  for (int32_t q = threadIdx.x; q < P; q += 256) {
+    int32_t rr = q, off = ob, li = 0;
+    bool ok = true;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int32_t gg = g0 + ty + 8 * j, ff = f0 + tx;
-    if (gg < ng && ff < nf) dst[ob + gg * d.o[g] + ff] = sg * tile[tx][ty + 8 * j];
+    for (int j = 0; j < PERM_MAXT; ++j) {
+      const int32_t qq = rr / wT[j];
+      const int32_t l = rr - qq * wT[j];
+      rr = qq;
+      ok = ok && l < wE[j];
+      off += l * wO[j];
+      li += l * wL[j];
+    }
+    if (ok) dst[off] = sg * lds[li];
   }
 }
 
@@ -1067,6 +1131,95 @@ static int koszul_negative(const int32_t *key, const int32_t *perm, int r) {
     for (int j = i + 1; j < r; ++j)
       if (perm[i] > perm[j]) neg ^= (key[perm[i]] & 1) & (key[perm[j]] & 1);
   return neg;
+}
+
+// Tile of a tile-mode block (the dimensions of d, merged, in output order,
+// with their input strides): returns the number of tiles, or 0 when no tile
+// of at most PERM_MAXT dimensions and PERM_LDS elements is found (the block
+// then takes the gather mode).  The input's innermost dimensions are taken
+// whole until a contiguous input run of `target` elements is reached (the
+// last one in part), likewise the output's; smaller targets are tried until
+// the tile fits; then the outermost partly taken dimension of either side is
+// doubled while the tile stays within PERM_LDS (fewer, larger workgroups).
+static int64_t choose_perm_tile(PDesc &d) {
+  const int nr = d.rank;
+  int in_ord[MAX_RANK], out_ord[MAX_RANK];
+  for (int k = 0; k < nr; ++k) in_ord[k] = out_ord[k] = k;
+  std::sort(in_ord, in_ord + nr, [&](int x, int y) { return d.s[x] < d.s[y]; });
+  std::reverse(out_ord, out_ord + nr);
+  int32_t T[MAX_RANK];
+  auto count = [&](int64_t &P) {
+    int nt = 0;
+    P = 1;
+    for (int k = 0; k < nr; ++k) {
+      P *= T[k];
+      if (T[k] > 1) ++nt;
+    }
+    return nt;
+  };
+  auto take = [&](const int *ord, int64_t target) {
+    int64_t run = 1;
+    for (int i = 0; i < nr; ++i) {
+      const int k = ord[i];
+      if (run * d.n[k] <= target) {
+        T[k] = d.n[k];
+        run *= d.n[k];
+        continue;
+      }
+      T[k] = std::max<int32_t>(T[k], (int32_t)std::min<int64_t>(d.n[k], (target + run - 1) / run));
+      return;
+    }
+  };
+  bool found = false;
+  int64_t P = 1;
+  for (int64_t target = 128; target >= PERM_RUN_MIN && !found; target /= 2) {
+    for (int k = 0; k < nr; ++k) T[k] = 1;
+    take(in_ord, target);
+    take(out_ord, target);
+    found = count(P) <= PERM_MAXT && P <= PERM_LDS;
+  }
+  if (!found) return 0;
+  // grow: the first not-whole dimension of each side, alternately
+  for (bool grew = true; grew && 2 * P <= PERM_LDS;) {
+    grew = false;
+    for (const int *ord : {in_ord, out_ord}) {
+      for (int i = 0; i < nr; ++i) {
+        const int k = ord[i];
+        if (T[k] == d.n[k]) continue;
+        const int32_t nt_k = std::min<int32_t>(d.n[k], std::max<int32_t>(2, 2 * T[k]));
+        int64_t P2 = 0;
+        const int32_t old = T[k];
+        T[k] = nt_k;
+        if (count(P2) <= PERM_MAXT && P2 <= PERM_LDS) {
+          P = P2;
+          grew = true;
+        } else {
+          T[k] = old;
+        }
+        break;
+      }
+    }
+  }
+  int64_t tiles = 1;
+  int nt = 0;
+  for (int k = 0; k < nr; ++k) {
+    d.t[k] = T[k];
+    d.c[k] = (d.n[k] + T[k] - 1) / T[k];
+    tiles *= d.c[k];
+  }
+  for (int j = 0; j < PERM_MAXT; ++j) d.rd[j] = d.wd[j] = -1, d.lt[j] = 0;
+  for (int i = 0, j = 0; i < nr; ++i)
+    if (T[in_ord[i]] > 1) d.rd[j++] = in_ord[i];
+  int32_t stride = 1;
+  for (int i = 0; i < nr; ++i)
+    if (T[out_ord[i]] > 1) {
+      d.wd[nt] = out_ord[i];
+      d.lt[nt] = stride;
+      stride *= T[out_ord[i]];
+      ++nt;
+    }
+  d.ptile = (int32_t)P;
+  return tiles;
 }
 
 static void build_transpose(TransposePlan &P, const tnx_blocks *a, const tnx_leg *legs, const int32_t *perm,
@@ -1150,16 +1303,11 @@ static void build_transpose(TransposePlan &P, const tnx_blocks *a, const tnx_leg
     d.size = (int32_t)acc;
     d.mode = 0;
     int64_t tasks = (acc + 256 * PERM_EPT - 1) / (256 * PERM_EPT);
-    if (acc > 0 && d.s[nr - 1] != 1) {
-      int gdim = -1;
-      for (int l = 0; l < nr; ++l)
-        if (d.s[l] == 1) gdim = l;
-      if (gdim >= 0 && d.n[gdim] >= PERM_TILE_MIN && d.n[nr - 1] >= PERM_TILE_MIN) {
+    if (acc > 0 && nr > 1 && !(d.s[nr - 1] == 1 && d.n[nr - 1] >= PERM_RUN)) {
+      const int64_t nt = choose_perm_tile(d);
+      if (nt > 0) {
         d.mode = 1;
-        d.g = gdim;
-        d.tg = (d.n[gdim] + 31) / 32;
-        d.tf = (d.n[nr - 1] + 31) / 32;
-        tasks = (acc / ((int64_t)d.n[gdim] * d.n[nr - 1])) * d.tg * d.tf;
+        tasks = nt;
       }
     }
     d.task0 = wg;
