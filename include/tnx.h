@@ -176,6 +176,27 @@ int tnx_transpose_execute(const tnx_plan *plan, const double *a, double *out,
 int tnx_plan_transpose_graded(const tnx_blocks *a, const tnx_leg *legs, const int32_t *perm,
                               int32_t graded, void *stream, tnx_plan **plan);
 
+/* Batched 2-D strided copy: desc[i] copies rows x cols float64 elements
+ * from src (row pitch src_ld) to dst (row pitch dst_ld), each multiplied by
+ * scale, all copies in one launch.  Assembles the sector matrices of the
+ * block-sparse SVD from the stored blocks and unpacks the kept singular
+ * vectors into the U / Vh blocks, in place of the reference's per-block
+ * slice updates (src/tenax/linalg.py:163-187 and 268-309).  desc is DEVICE
+ * memory (n records); copy i is done by workgroups task0 .. task0 +
+ * ceil(rows * cols / TNX_COPY2D_CHUNK) - 1, task0 being the sum of those
+ * counts over copies 0 .. i-1, and n_wg the sum over all copies.  Ranges may
+ * not overlap between destinations; src and dst must not overlap. */
+#define TNX_COPY2D_CHUNK 4096
+typedef struct {
+  const double *src;
+  double *dst;
+  int64_t src_ld, dst_ld;
+  int64_t task0;
+  int32_t rows, cols;
+  double scale;
+} tnx_copy2d;
+int tnx_copy2d_execute(const tnx_copy2d *desc, int32_t n, int64_t n_wg, void *stream);
+
 /* Work-list validation (no reference counterpart; a debug and test aid).
  * tnx_plan_validate checks every grouped-GEMM work list of a plan (pairwise,
  * matvec or environment; default and tiled task lists) and the matvec's mix

@@ -90,6 +90,7 @@ SIGNATURES = {
     "tnx_contract_vjp": (C.c_int, [_p, _p, _p, _p, _p, _p, _i32, _p]),
     "tnx_plan_transpose": (C.c_int, [C.POINTER(Blocks), C.POINTER(Leg), _PI32, _p, C.POINTER(_p)]),
     "tnx_transpose_execute": (C.c_int, [_p, _p, _p, _p]),
+    "tnx_copy2d_execute": (C.c_int, [_p, _i32, _i64, _p]),
     "tnx_plan_validate": (C.c_int, [_p, _p]),
     "tnx_check_gemm_worklist": (C.c_int, [_i32, _p, _i32, _p, _i32, _p, _i32, _i32, _i32, _i64, _PI64, _i32,
                                           _PI64, _i32, _PI64, _i32]),
@@ -247,3 +248,36 @@ class Plan:
         npairs, ntasks = C.c_int32(), C.c_int32()
         check(_lib.tnx_plan_stats(self.ptr, C.byref(f), C.byref(b), C.byref(npairs), C.byref(ntasks)))
         return {"flops": f.value, "bytes": b.value, "n_pairs": npairs.value, "n_tasks": ntasks.value}
+
+
+COPY2D_CHUNK = 4096  # TNX_COPY2D_CHUNK (include/tnx.h)
+_COPY2D = np.dtype([("src", np.uint64), ("dst", np.uint64), ("src_ld", np.int64), ("dst_ld", np.int64),
+                    ("task0", np.int64), ("rows", np.int32), ("cols", np.int32), ("scale", np.float64)])
+
+
+def copy2d(copies, device):
+    """Batched 2-D strided copies in one launch (tnx_copy2d_execute).
+
+    copies: iterable of (src, src_off, src_ld, dst, dst_off, dst_ld, rows,
+    cols, scale) with src / dst contiguous float64 device tensors and element
+    offsets / row pitches; each rectangle is checked to lie inside its tensor.
+    """
+    import torch
+
+    copies = [c for c in copies if c[6] > 0 and c[7] > 0]
+    if not copies:
+        return
+    d = np.zeros(len(copies), _COPY2D)
+    wg = 0
+    for i, (src, so, sld, dst, do, dld, rows, cols, scale) in enumerate(copies):
+        for t, off, ld in ((src, so, sld), (dst, do, dld)):
+            if t.dtype != torch.float64 or not t.is_contiguous():
+                raise ValueError("copy2d needs contiguous float64 tensors")
+            if off < 0 or ld < cols or off + (rows - 1) * ld + cols > t.numel():
+                raise ValueError(f"copy2d rectangle {rows}x{cols} at {off} (pitch {ld}) outside a tensor of "
+                                 f"{t.numel()} elements")
+        d[i] = (src.data_ptr() + 8 * so, dst.data_ptr() + 8 * do, sld, dld, wg, rows, cols, scale)
+        wg += -(-rows * cols // COPY2D_CHUNK)
+    dev = torch.from_numpy(d.view(np.uint8)).to(device)
+    check(load().tnx_copy2d_execute(dptr(dev), len(copies), wg, stream_handle()))
+    return dev  # the caller may keep it alive until the copies ran (stream order makes this optional)

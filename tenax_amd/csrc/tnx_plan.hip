@@ -1122,6 +1122,31 @@ __global__ __launch_bounds__(256) void permute_kernel(const PDesc *__restrict__ 
   }
 }
 
+// Batched 2-D strided copy (tnx_copy2d_execute): a workgroup copies one
+// TNX_COPY2D_CHUNK-element range of one copy's row-major rows x cols
+// rectangle, consecutive threads on consecutive elements of a row.
+static_assert(sizeof(tnx_copy2d) == 56, "tnx_copy2d layout");
+__global__ __launch_bounds__(256) void copy2d_kernel(const tnx_copy2d *__restrict__ desc, int32_t n) {
+  const int64_t w = blockIdx.x;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (desc[mid].task0 <= w) lo = mid; else hi = mid - 1;
+  }
+  const tnx_copy2d &d = desc[lo];
+  const int64_t size = (int64_t)d.rows * d.cols;
+  const int64_t e0 = (w - d.task0) * TNX_COPY2D_CHUNK;
+  const double sc = d.scale;
+#pragma unroll 4
+  for (int j = 0; j < TNX_COPY2D_CHUNK / 256; ++j) {
+    const int64_t e = e0 + threadIdx.x + 256 * j;
+    if (e < size) {
+      const int64_t r = e / d.cols, c = e - r * d.cols;
+      d.dst[r * d.dst_ld + c] = sc * d.src[r * d.src_ld + c];
+    }
+  }
+}
+
 // Koszul sign of a block under a leg permutation (tensor.py:32-50, 853-856):
 // -1 per inversion (i < j, perm[i] > perm[j]) whose two legs carry odd
 // charges (parity = charge & 1).
@@ -1548,6 +1573,17 @@ int tnx_plan_transpose_graded(const tnx_blocks *a, const tnx_leg *legs, const in
     std::unique_ptr<TransposePlan> P(new TransposePlan());
     build_transpose(*P, a, legs, perm, (hipStream_t)stream, graded != 0);
     *plan = P.release();
+  });
+}
+
+int tnx_copy2d_execute(const tnx_copy2d *desc, int32_t n, int64_t n_wg, void *stream) {
+  return guarded([&] {
+    TNX_REQUIRE(n >= 0 && n_wg >= 0, TNX_ERR_ARG, "negative count");
+    if (n == 0 || n_wg == 0) return;
+    TNX_REQUIRE(desc, TNX_ERR_ARG, "null descriptor array");
+    TNX_REQUIRE(n_wg < (int64_t(1) << 31), TNX_ERR_CAPACITY, "too many copy workgroups");
+    hipLaunchKernelGGL(copy2d_kernel, dim3((unsigned)n_wg), dim3(256), 0, (hipStream_t)stream, desc, n);
+    TNX_CHECK_HIP(hipGetLastError());
   });
 }
 

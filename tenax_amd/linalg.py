@@ -16,7 +16,12 @@ Sector visit order here: first appearance in the sorted block order (the
 reference visits dict insertion order; only exact ties across sectors can
 tell the two apart).  Row/column sub-keys inside a sector are likewise in
 sorted order.  Blocks enter the sector matrix reshaped in their stored leg
-order, exactly as the reference's linalg.py:181 does (see _assemble).
+order, exactly as the reference's linalg.py:181 does (see _assemble).  All
+sector matrices of a call live in one device buffer filled by one batched
+copy launch (tnx_copy2d_execute), and the kept singular vectors are unpacked
+into the U and Vh blocks by one launch each, where the reference updates one
+block slice at a time (linalg.py:163-187, 268-309); only the spectrum and
+its sort order travel to the host, which fixes the new bond's structure.
 
 Decomposition.  Without ``absorb`` each sector matrix goes through
 torch.linalg.svd (rocSOLVER gesvd) in float64, and so does the DMRG split
@@ -49,6 +54,7 @@ from collections.abc import Sequence
 import numpy as np
 import torch
 
+from tenax_amd import _lib
 from tenax_amd.core.index import FlowDirection, Label, TensorIndex
 from tenax_amd.core.tensor import DenseTensor, SymmetricTensor, Tensor
 
@@ -87,18 +93,6 @@ def _sector_layout(t: SymmetricTensor, lax, rax):
             c0 += nc
         sec["R"], sec["C"] = r0, c0
     return sectors
-
-
-def _pack(indices, items, device):
-    """items: list of (key tuple, torch block); returns packed SymmetricTensor."""
-    items = sorted(items, key=lambda kv: kv[0])
-    r = len(indices)
-    keys = np.array([k for k, _ in items], np.int32).reshape(len(items), r)
-    shapes = np.array([tuple(b.shape) for _, b in items], np.int32).reshape(len(items), r)
-    offs, tot = SymmetricTensor._layout(shapes)
-    data = torch.cat([b.reshape(-1) for _, b in items]) if items else torch.zeros(0, dtype=torch.float64,
-                                                                                   device=device)
-    return SymmetricTensor._wrap(indices, keys, shapes, offs, data.contiguous())
 
 
 # rocSOLVER's batched eigh costs about the same per call for a batch of 1 or
@@ -599,25 +593,30 @@ def _assemble(t: SymmetricTensor, lax, rax):
     sectors = _sector_layout(t, lax, rax)
     sym = t.indices[0].symmetry
     perm = list(lax) + list(rax)
-    q_list, mats = [], []
+    q_list, dims = [], []
     for q, sec in sectors.items():
-        R, C = sec["R"], sec["C"]
-        if R == 0 or C == 0:
-            continue
-        M = torch.zeros((R, C), dtype=torch.float64, device=t.data.device)
+        if sec["R"] and sec["C"]:
+            q_list.append(q)
+            dims.append((sec["R"], sec["C"]))
+    if not q_list:
+        raise ValueError("SVD of a tensor with no blocks")
+    # all sector matrices in one zeroed buffer, the blocks copied in by one launch
+    starts = np.concatenate([[0], np.cumsum([R * C for R, C in dims])]).astype(np.int64)
+    buf = torch.zeros(int(starts[-1]), dtype=torch.float64, device=t.data.device)
+    data = t.data.detach()
+    copies = []
+    for j, q in enumerate(q_list):
+        sec = sectors[q]
+        C = dims[j][1]
         for i, lk, rk in sec["blocks"]:
             nr = sec["rows"][lk][0]
             nc = sec["cols"][rk][0]
-            r0, c0 = sec["row_off"][lk], sec["col_off"][rk]
-            blk = t.block(i).reshape(nr, nc)
-            if sym.is_fermionic and _koszul_negative(sym.parity(t.keys[i]), perm):
-                # Koszul sign of the reordering to (left, right) legs (linalg.py:171-186)
-                blk = -blk
-            M[r0:r0 + nr, c0:c0 + nc] = blk
-        mats.append(M)
-        q_list.append(q)
-    if not mats:
-        raise ValueError("SVD of a tensor with no blocks")
+            # Koszul sign of the reordering to (left, right) legs (linalg.py:171-186)
+            neg = sym.is_fermionic and _koszul_negative(sym.parity(t.keys[i]), perm)
+            copies.append((data, int(t.offsets[i]), nc, buf, int(starts[j]) + sec["row_off"][lk] * C
+                           + sec["col_off"][rk], C, nr, nc, -1.0 if neg else 1.0))
+    _lib.copy2d(copies, buf.device)
+    mats = [buf[starts[j]:starts[j + 1]].view(R, C) for j, (R, C) in enumerate(dims)]
     return sectors, q_list, mats
 
 
@@ -632,6 +631,9 @@ def truncated_svd_symmetric_many(tensors, left_labels, right_labels, new_bond_la
     truncation-error rule): only the leading max_singular_values triplets of
     each sector are computed (block subspace iteration); the returned full
     spectrum then holds those values only."""
+    if torch.is_grad_enabled() and any(t.data.requires_grad for t in tensors):
+        raise NotImplementedError("autograd through the block-sparse SVD is not supported (the CTM and DMRG "
+                                  "paths factorise constants)")
     prepared = []
     all_mats = []
     for t in tensors:
@@ -732,7 +734,7 @@ def _truncate_and_pack(t, lax, rax, sectors, q_list, mats, factors, new_bond_lab
     sym = t.indices[0].symmetry
     bond_out = TensorIndex(sym, bond_charges, FlowDirection.OUT, new_bond_label)
     bond_in = TensorIndex(sym, bond_charges, FlowDirection.IN, new_bond_label)
-    U_items, V_items = [], []
+    U_items, V_items = [], []  # (key, block shape, rows, cols, factor, first row, first column, scale)
     for j, q in enumerate(q_list):
         nq = int(n_per[j])
         if nq == 0:
@@ -749,21 +751,51 @@ def _truncate_and_pack(t, lax, rax, sectors, q_list, mats, factors, new_bond_lab
             Uq = _orthonormal_columns(Uq)
             Vq = _orthonormal_columns(Vq.T).T
         # with absorb the Gram factors already carry s on the absorbing side
+        su = sv = 1.0
         if normalize and absorb in ("left", "right") and s_sum > 0:
             if absorb == "left":
-                Uq = Uq / s_sum
+                su = 1.0 / s_sum
             else:
-                Vq = Vq / s_sum
+                sv = 1.0 / s_sum
         sec = sectors[q]
         for lk, (nr, shp) in sec["rows"].items():
-            r0 = sec["row_off"][lk]
-            U_items.append((lk + (q,), Uq[r0:r0 + nr, :].reshape(shp + (nq,))))
+            U_items.append((lk + (q,), shp + (nq,), nr, nq, Uq, sec["row_off"][lk], 0, su))
         for rk, (nc, shp) in sec["cols"].items():
-            c0 = sec["col_off"][rk]
-            V_items.append(((q,) + rk, Vq[:, c0:c0 + nc].reshape((nq,) + shp)))
+            V_items.append(((q,) + rk, (nq,) + shp, nq, nc, Vq, 0, sec["col_off"][rk], sv))
     U_idx = tuple(t.indices[a] for a in lax) + (bond_out,)
     V_idx = (bond_in,) + tuple(t.indices[a] for a in rax)
-    return _pack(U_idx, U_items, dev), s_final, _pack(V_idx, V_items, dev), s_sorted
+    U_t, V_t = _pack_copies(U_idx, U_items, dev), _pack_copies(V_idx, V_items, dev)
+    return U_t, s_final, V_t, s_sorted
+
+
+def _addressable(X):
+    """(1-D view over the whole storage of X, element offset of X[0, 0] in it,
+    row pitch) for a 2-D float64 X whose columns are contiguous (a copy of X
+    is made when they are not)."""
+    if X.shape[1] > 1 and X.stride(1) != 1:
+        X = X.contiguous()
+    pitch = X.stride(0) if X.shape[0] > 1 else X.shape[1]
+    n = X.untyped_storage().nbytes() // X.element_size()
+    return X.as_strided((n,), (1,), 0), X.storage_offset(), pitch
+
+
+def _pack_copies(indices, items, device):
+    """Packed SymmetricTensor of blocks given as sub-rectangles of factor
+    matrices: items (key, block shape, rows, cols, source matrix, first row,
+    first column, scale); the blocks are laid out in sorted key order and
+    filled by one batched copy."""
+    items = sorted(items, key=lambda it: it[0])
+    r = len(indices)
+    keys = np.array([it[0] for it in items], np.int32).reshape(len(items), r)
+    shapes = np.array([it[1] for it in items], np.int32).reshape(len(items), r)
+    offs, tot = SymmetricTensor._layout(shapes)
+    data = torch.empty(tot, dtype=torch.float64, device=device)
+    copies = []
+    for (key, shp, rows, cols, X, r0, c0, scale), off in zip(items, offs):
+        base, xoff, pitch = _addressable(X)
+        copies.append((base, xoff + r0 * pitch + c0, pitch, data, int(off), cols, rows, cols, scale))
+    _lib.copy2d(copies, data.device)
+    return SymmetricTensor._wrap(indices, keys, shapes, offs, data)
 
 
 def svd(tensor: Tensor, left_labels: Sequence[Label], right_labels: Sequence[Label], new_bond_label: Label = "bond",

@@ -130,3 +130,27 @@ def test_non_leading_left_legs_follow_reference_reshape(gpu, theta10, absorb):
     ref = np.tensordot(Uo.todense() * so, Vo.todense(), axes=([2], [0]))
     got = _dense_product(U, s.cpu().numpy(), Vh, None)
     assert np.abs(got - ref).max() < 1e-11 * smax
+
+
+def test_copy2d_batched_rectangles(gpu):
+    """tnx_copy2d_execute: many strided rectangles (1x1 up to chunk-spanning
+    ones, scaled) in one launch, against torch slicing; nothing outside the
+    destination rectangles is touched."""
+    from tenax_amd import _lib
+
+    rng = np.random.default_rng(5)
+    src = torch.tensor(rng.standard_normal(200_000), device="cuda")
+    dst = torch.full((300_000,), 7.0, dtype=torch.float64, device="cuda")
+    ref = dst.clone()
+    copies, at = [], 0
+    for rows, cols in [(1, 1), (3, 5), (17, 300), (100, 41), (1, 5000), (64, 64), (7, 1)]:
+        sld = cols + int(rng.integers(0, 9))
+        so = int(rng.integers(0, src.numel() - rows * sld))
+        dld = cols + int(rng.integers(0, 5))
+        scale = float(rng.choice([1.0, -1.0, 0.25]))
+        copies.append((src, so, sld, dst, at, dld, rows, cols, scale))
+        for r in range(rows):
+            ref[at + r * dld: at + r * dld + cols] = scale * src[so + r * sld: so + r * sld + cols]
+        at += rows * dld + 3
+    _lib.copy2d(copies, dst.device)
+    assert torch.equal(dst, ref)

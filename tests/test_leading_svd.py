@@ -92,9 +92,12 @@ def test_rank_deficient_blocks_are_completed(kind):
     assert np.max(np.abs(U.T @ low @ Vh.T - np.diag(s))) <= 1e-13 * se[0]
 
 
-def test_truncated_svd_leading_mode_equals_full_mode():
+@pytest.mark.gpu
+def test_truncated_svd_leading_mode_equals_full_mode(gpu):
     """truncated_svd_symmetric_many(leading=True) keeps the same singular
-    values and spans as the full Jordan-Wielandt path on a Z2 tensor."""
+    values and spans as the full Jordan-Wielandt path on a Z2 tensor (on the
+    GPU: the sector matrices are assembled and unpacked by the device copy
+    kernel)."""
     from tenax_amd.core.index import FlowDirection, TensorIndex
     from tenax_amd.core.symmetry import ZnSymmetry
     from tenax_amd.core.tensor import SymmetricTensor
@@ -113,7 +116,7 @@ def test_truncated_svd_leading_mode_equals_full_mode():
     shapes = np.array(shapes, np.int32)
     offs, tot = SymmetricTensor._layout(shapes)
     # a decaying spectrum: blocks scaled by a smooth profile
-    data = torch.tensor(rng.standard_normal(tot) * np.exp(-np.arange(tot) / (tot / 8)))
+    data = torch.tensor(rng.standard_normal(tot) * np.exp(-np.arange(tot) / (tot / 8)), device="cuda")
     t = SymmetricTensor._wrap(idx, keys, shapes, offs, data)
     full = L.truncated_svd_symmetric_many([t], ["a", "b"], ["c", "d"], "x", 20, absorb="isometries")[0]
     lead = L.truncated_svd_symmetric_many([t], ["a", "b"], ["c", "d"], "x", 20, absorb="isometries",

@@ -37,3 +37,18 @@ def test_version_and_error_text():
     assert lib.tnx_version() >= 1000
     buf = ctypes.create_string_buffer(64)
     assert lib.tnx_last_error(buf, 64) == 0
+
+
+def test_copy2d_rejects_rectangles_outside_their_tensors():
+    """The batched copy's host-side checks run before any device work."""
+    import pytest
+    import torch
+
+    a = torch.zeros(100, dtype=torch.float64)
+    b = torch.zeros(100, dtype=torch.float64)
+    with pytest.raises(ValueError, match="outside a tensor"):
+        _lib.copy2d([(a, 90, 10, b, 0, 10, 2, 10, 1.0)], a.device)  # source rows 90..109
+    with pytest.raises(ValueError, match="outside a tensor"):
+        _lib.copy2d([(a, 0, 10, b, 0, 5, 2, 10, 1.0)], a.device)  # pitch below the row length
+    with pytest.raises(ValueError, match="contiguous float64"):
+        _lib.copy2d([(a.float(), 0, 10, b, 0, 10, 1, 10, 1.0)], a.device)
