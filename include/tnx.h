@@ -176,6 +176,25 @@ int tnx_transpose_execute(const tnx_plan *plan, const double *a, double *out,
 int tnx_plan_transpose_graded(const tnx_blocks *a, const tnx_leg *legs, const int32_t *perm,
                               int32_t graded, void *stream, tnx_plan **plan);
 
+/* Many independent instances in one launch (the many-temperature HOTRG step,
+ * src/tenax/algorithms/hotrg.py:109-215, run for every instance at once; the
+ * reference contracts each instance with its own per-block einsums).
+ * tnx_batch_pairwise binds pairwise plans[i] to its operands a[i], b[i] and
+ * output c[i] (device buffers laid out as the plan's operands and output) and
+ * merges their block pairs into one grouped-GEMM program (default 64 x 64
+ * tiles, longest K first over all instances); tnx_batch_transpose does the
+ * same for transpose plans (in[i] -> out[i]).  The outputs must not overlap
+ * each other; all plans of one pairwise batch must share the A layout
+ * (a_kfirst).  tnx_batch_execute runs a batch (compute as in
+ * tnx_contract_execute; ignored for transposes).  A batch is a plan object:
+ * it keeps the buffer addresses it was built with and is freed with
+ * tnx_plan_free; the instance plans may be freed after the batch is built. */
+int tnx_batch_pairwise(const tnx_plan *const *plans, int32_t n, const double *const *a,
+                       const double *const *b, double *const *c, void *stream, tnx_plan **batch);
+int tnx_batch_transpose(const tnx_plan *const *plans, int32_t n, const double *const *in,
+                        double *const *out, void *stream, tnx_plan **batch);
+int tnx_batch_execute(const tnx_plan *batch, int32_t compute, void *stream);
+
 /* Batched 2-D strided copy: desc[i] copies rows x cols float64 elements
  * from src (row pitch src_ld) to dst (row pitch dst_ld), each multiplied by
  * scale, all copies in one launch.  Assembles the sector matrices of the

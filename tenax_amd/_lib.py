@@ -91,6 +91,10 @@ SIGNATURES = {
     "tnx_plan_transpose": (C.c_int, [C.POINTER(Blocks), C.POINTER(Leg), _PI32, _p, C.POINTER(_p)]),
     "tnx_transpose_execute": (C.c_int, [_p, _p, _p, _p]),
     "tnx_copy2d_execute": (C.c_int, [_p, _i32, _i64, _p]),
+    "tnx_batch_pairwise": (C.c_int, [C.POINTER(_p), _i32, C.POINTER(_p), C.POINTER(_p), C.POINTER(_p), _p,
+                                     C.POINTER(_p)]),
+    "tnx_batch_transpose": (C.c_int, [C.POINTER(_p), _i32, C.POINTER(_p), C.POINTER(_p), _p, C.POINTER(_p)]),
+    "tnx_batch_execute": (C.c_int, [_p, _i32, _p]),
     "tnx_plan_validate": (C.c_int, [_p, _p]),
     "tnx_check_gemm_worklist": (C.c_int, [_i32, _p, _i32, _p, _i32, _p, _i32, _i32, _i32, _i64, _PI64, _i32,
                                           _PI64, _i32, _PI64, _i32]),

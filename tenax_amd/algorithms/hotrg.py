@@ -19,8 +19,10 @@ All contractions are block-sparse contractions of the device engine.  For
 SymmetricTensor inputs the SVD is the batched Jordan-Wielandt eigh
 factorisation (``absorb="isometries"`` in tenax_amd.linalg: both isometries
 with the SVD's pairing and LAPACK-level accuracy for small singular values),
-and ``hotrg_batched`` runs every instance's step in lockstep so the sector
-eigenproblems of all instances share one batched eigh per step.
+and ``hotrg_batched`` runs every instance's step in lockstep: the sector
+eigenproblems of all instances share one batched eigh per step, and each
+block contraction of the step is one grouped-GEMM launch over the block pairs
+of every instance (contract_many).
 """
 
 from __future__ import annotations
@@ -31,7 +33,7 @@ from dataclasses import dataclass
 import torch
 
 from tenax_amd.algorithms._tensor_utils import max_abs_normalize
-from tenax_amd.contraction import contract
+from tenax_amd.contraction import contract_many
 from tenax_amd.core.tensor import SymmetricTensor, Tensor
 from tenax_amd.linalg import truncated_svd, truncated_svd_symmetric_many
 
@@ -94,25 +96,55 @@ def _isometries(Ms, roles, max_bond_dim, svd_trunc_err):
 
 
 def _steps(Ts, roles, max_bond_dim, svd_trunc_err):
+    """One coarse-graining step of every instance, in lockstep: each
+    contraction runs for all instances at once (contract_many: one grouped
+    block-GEMM launch over every instance's block pairs, one launch per leg
+    permutation phase)."""
     t = _tick("", 0.0)
-    Ms = [contract(T, T.relabels(roles["env"])) for T in Ts]
+    Ms = contract_many([(T, T.relabels(roles["env"])) for T in Ts])
     t = _tick("environment", t)
     isos = _isometries(Ms, roles, max_bond_dim, svd_trunc_err)
     t = _tick("svd", t)
     del Ms
+    Uds = [U.dagger() for U, _ in isos]
+    Vbs = [Vh.relabel("a", "b").dagger() for _, Vh in isos]
     out = []
-    for T, (U, Vh) in zip(Ts, isos):
-        Ud = U.dagger()
-        Vb = Vh.relabel("a", "b").dagger()
-        first = T.relabel(*roles["first"])
-        second = T.relabels(roles["second"])
-        X = contract(Ud, first)
-        Y = contract(X, second)
+    firsts = [T.relabel(*roles["first"]) for T in Ts]
+    for lo, hi in _chunks(list(zip(Uds, firsts))):
+        X = contract_many(list(zip(Uds[lo:hi], firsts[lo:hi])))
+        Y = contract_many([(x, T.relabels(roles["second"])) for x, T in zip(X, Ts[lo:hi])])
         del X
-        Tn = contract(Y, Vb, output_labels=roles["out"])
+        Tn = contract_many(list(zip(Y, Vbs[lo:hi])), output_labels=roles["out"])
         del Y
-        out.append(max_abs_normalize(Tn.relabels(roles["back"])))
+        out.extend(max_abs_normalize(T.relabels(roles["back"])) for T in Tn)
     _tick("coarse_grain", t)
+    return out
+
+
+# Instances whose chi^5 intermediates are contracted together: at most this
+# many elements of the dense bound (the stored Z2 blocks hold about half),
+# so that the intermediates of one group and their permuted copies stay
+# within a few tens of GB of HBM.
+CHUNK_ELEMS = 1 << 32
+
+
+def _chunks(pairs):
+    """[lo, hi) ranges of instances whose first intermediates (the dense size
+    of the open legs of each pair) fit CHUNK_ELEMS together; every range holds
+    at least one instance."""
+    out, lo, acc = [], 0, 0
+    for i, (A, B) in enumerate(pairs):
+        la = {ix.label: ix.dim for ix in A.indices}
+        lb = {ix.label: ix.dim for ix in B.indices}
+        n = 1
+        for lbl, d in list(la.items()) + list(lb.items()):
+            if (lbl in la) != (lbl in lb):
+                n *= d
+        if i > lo and acc + n > CHUNK_ELEMS:
+            out.append((lo, i))
+            lo, acc = i, 0
+        acc += n
+    out.append((lo, len(pairs)))
     return out
 
 

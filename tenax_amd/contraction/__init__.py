@@ -4,6 +4,7 @@ from tenax_amd.contraction.contractor import (
     blockwise_contract,
     compute_mode,
     contract,
+    contract_many,
     contract_with_subscripts,
     get_compute,
     per_block_execution,
@@ -11,5 +12,5 @@ from tenax_amd.contraction.contractor import (
     set_compute,
 )
 
-__all__ = ["contract", "contract_with_subscripts", "blockwise_contract", "set_compute", "get_compute",
+__all__ = ["contract", "contract_many", "contract_with_subscripts", "blockwise_contract", "set_compute", "get_compute",
            "compute_mode", "per_block_execution", "plan_order"]

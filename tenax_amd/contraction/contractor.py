@@ -232,6 +232,66 @@ def _is_graded(T) -> bool:
     return bool(T.indices) and T.indices[0].symmetry.is_fermionic
 
 
+class _PairJob:
+    """The host-side decisions of one pairwise contraction: which operand
+    plays A, the leg permutations that bring A to [free, contracted] (or
+    [contracted, free]) and B to [contracted, free], the planner's arguments
+    and the permutation of the GEMM's natural output order to the requested
+    one."""
+
+    __slots__ = ("A", "B", "a_axes", "b_axes", "a_kfirst", "nc", "nat", "nat_indices", "out_axes", "mode",
+                 "target", "sign_pairs", "graded")
+
+    def __init__(self, A, B, sa, sb, so, out_indices, mode, target):
+        if len(set(sa)) != len(sa) or len(set(sb)) != len(sb):
+            raise NotImplementedError("traces within one operand are not supported")
+        cont = [c for c in sa if c in sb]
+        free_a = [c for c in sa if c not in cont]
+        free_b = [c for c in sb if c not in cont]
+        if sorted(so) != sorted(free_a + free_b):
+            raise ValueError(f"output subscripts {so!r} are not the free legs of {sa!r},{sb!r}")
+        self.graded = _is_graded(A)
+        sign_chars = []
+        if self.graded:
+            if mode != _lib.MODE_PAIRWISE_FILTERED:
+                raise NotImplementedError("fermionic signs are defined for the contract() semantics only")
+            if torch.is_grad_enabled() and (A.data.requires_grad or B.data.requires_grad):
+                raise NotImplementedError("autograd through fermionic block contractions is not supported")
+            sign_chars = _inversion_pairs([sa, sb], so)
+        ix_of = dict(zip(so, out_indices))
+        # Let the operand whose free leg comes first in the output play "A".
+        if free_a and free_b and so[0] in free_b:
+            A, B, sa, sb = B, A, sb, sa
+            cont = [c for c in sa if c in sb]
+            free_a, free_b = free_b, free_a
+        fa = [c for c in so if c in free_a]
+        fb = [c for c in so if c in free_b]
+        self.nat = "".join(fa + fb)
+        # A layout
+        self.a_axes = None
+        if list(sa) == fa + cont:
+            self.a_kfirst = 0
+        elif list(sa) == cont + fa:
+            self.a_kfirst = 1
+        else:
+            self.a_axes = tuple(sa.index(c) for c in fa + cont)
+            sa, self.a_kfirst = "".join(fa + cont), 0
+        # B layout: always [contracted (in A's order)..., free...]
+        self.b_axes = None
+        if list(sb) != cont + fb:
+            self.b_axes = tuple(sb.index(c) for c in cont + fb)
+            sb = "".join(cont + fb)
+        self.A, self.B = A, B
+        self.nc = len(cont)
+        self.nat_indices = tuple(ix_of[c] for c in self.nat)
+        pos = {c: i for i, c in enumerate(sa)}
+        for i, c in enumerate(sb):
+            pos.setdefault(c, len(sa) + i)
+        self.sign_pairs = [(pos[x], pos[y]) for x, y in sign_chars]
+        self.out_axes = tuple(self.nat.index(c) for c in so) if self.nat != so else None
+        self.mode, self.target = mode, target
+
+
 def contract_pair(A: SymmetricTensor, B: SymmetricTensor, sa: str, sb: str, so: str,
                   out_indices: tuple[TensorIndex, ...], mode: int, target) -> SymmetricTensor:
     """One pairwise block-sparse contraction with einsum-style subscripts.
@@ -242,63 +302,136 @@ def contract_pair(A: SymmetricTensor, B: SymmetricTensor, sa: str, sb: str, so: 
     the reference (contractor.py:462-470), evaluated on the device; the
     operand and output reshuffles below then move data without signs.
     """
-    if len(set(sa)) != len(sa) or len(set(sb)) != len(sb):
-        raise NotImplementedError("traces within one operand are not supported")
-    cont = [c for c in sa if c in sb]
-    free_a = [c for c in sa if c not in cont]
-    free_b = [c for c in sb if c not in cont]
-    if sorted(so) != sorted(free_a + free_b):
-        raise ValueError(f"output subscripts {so!r} are not the free legs of {sa!r},{sb!r}")
-    graded = _is_graded(A)
-    sign_chars = []
-    if graded:
-        if mode != _lib.MODE_PAIRWISE_FILTERED:
-            raise NotImplementedError("fermionic signs are defined for the contract() semantics only")
-        if torch.is_grad_enabled() and (A.data.requires_grad or B.data.requires_grad):
-            raise NotImplementedError("autograd through fermionic block contractions is not supported")
-        sign_chars = _inversion_pairs([sa, sb], so)
-    ix_of = dict(zip(so, out_indices))
     if not so:
+        if len(set(sa)) != len(sa) or len(set(sb)) != len(sb):
+            raise NotImplementedError("traces within one operand are not supported")
+        if sorted(sa) != sorted(sb):
+            raise ValueError(f"output subscripts {so!r} are not the free legs of {sa!r},{sb!r}")
+        sign_chars = _inversion_pairs([sa, sb], so) if _is_graded(A) else []
         return _full_contraction(A, B, sa, sb, sign_chars)
-    # Let the operand whose free leg comes first in the output play "A".
-    if free_a and free_b and so[0] in free_b:
-        A, B, sa, sb = B, A, sb, sa
-        cont = [c for c in sa if c in sb]
-        free_a, free_b = free_b, free_a
-    fa = [c for c in so if c in free_a]
-    fb = [c for c in so if c in free_b]
-    nat = "".join(fa + fb)
+    job = _PairJob(A, B, sa, sb, so, out_indices, mode, target)
+    graded = job.graded
     move = (lambda T, axes: _transpose_raw(T, axes, graded=False)) if graded else (lambda T, axes: T.transpose(axes))
-    # A layout
-    if list(sa) == fa + cont:
-        a_kfirst = 0
-    elif list(sa) == cont + fa:
-        a_kfirst = 1
-    else:
-        A = move(A, tuple(sa.index(c) for c in fa + cont))
-        sa, a_kfirst = "".join(fa + cont), 0
-    # B layout: always [contracted (in A's order)..., free...]
-    b_kfirst = 1
-    if list(sb) != cont + fb:
-        B = move(B, tuple(sb.index(c) for c in cont + fb))
-        sb = "".join(cont + fb)
-    nat_indices = tuple(ix_of[c] for c in nat)
-    pos = {c: i for i, c in enumerate(sa)}
-    for i, c in enumerate(sb):
-        pos.setdefault(c, len(sa) + i)
-    plan, keys, shapes, offs, tot = plans.pairwise_plan(A, B, a_kfirst, b_kfirst, len(cont), nat_indices, mode,
-                                                        target, [(pos[x], pos[y]) for x, y in sign_chars])
+    A = job.A if job.a_axes is None else move(job.A, job.a_axes)
+    B = job.B if job.b_axes is None else move(job.B, job.b_axes)
+    plan, keys, shapes, offs, tot = plans.pairwise_plan(A, B, job.a_kfirst, 1, job.nc, job.nat_indices, mode,
+                                                        target, job.sign_pairs)
     out = torch.empty(tot, dtype=torch.float64, device=A.data.device)
-    C = SymmetricTensor._wrap(nat_indices, keys, shapes, offs, out)
+    C = SymmetricTensor._wrap(job.nat_indices, keys, shapes, offs, out)
     if torch.is_grad_enabled() and (A.data.requires_grad or B.data.requires_grad):
         Ad = A._with_data(A.data.detach())
         Bd = B._with_data(B.data.detach())
-        C = C._with_data(_PairExecFn.apply(A.data, B.data, plan, Ad, Bd, C, sa, sb, nat))
+        C = C._with_data(_PairExecFn.apply(A.data, B.data, plan, Ad, Bd, C, None, None, job.nat))
     elif tot:
         _execute_pair(plan, A, B, out)
-    if nat != so:
-        C = move(C, tuple(nat.index(c) for c in so))
+    if job.out_axes is not None:
+        C = move(C, job.out_axes)
     return C
+
+
+def _batched_transposes(tensors, axes_list):
+    """Leg permutations of several tensors (no signs) in one launch
+    (tnx_batch_transpose); returns the permuted tensors, their data
+    consecutive in one buffer."""
+    lib = _lib.load()
+    items = [plans.transpose_plan(T, axes, False) for T, axes in zip(tensors, axes_list)]
+    starts = np.concatenate([[0], np.cumsum([it[4] for it in items])]).astype(np.int64)
+    dev = tensors[0].data.device
+    buf = torch.empty(int(starts[-1]), dtype=torch.float64, device=dev)
+    outs = [buf[starts[k]:starts[k + 1]] for k in range(len(items))]
+    n = len(items)
+    P = (_lib.C.c_void_p * n)(*[it[0].ptr for it in items])
+    I = (_lib.C.c_void_p * n)(*[_lib.dptr(T.data.detach()) for T in tensors])
+    O = (_lib.C.c_void_p * n)(*[_lib.dptr(o) for o in outs])
+    batch = _lib.C.c_void_p()
+    _lib.check(lib.tnx_batch_transpose(P, n, I, O, _lib.stream_handle(), _lib.C.byref(batch)))
+    bplan = _lib.Plan(batch)
+    _lib.check(lib.tnx_batch_execute(bplan.ptr, 0, _lib.stream_handle()))
+    return [SymmetricTensor._wrap(tuple(T.indices[i] for i in axes), it[1], it[2], it[3], o)
+            for T, axes, it, o in zip(tensors, axes_list, items, outs)]
+
+
+def contract_pairs_batched(jobs) -> list[SymmetricTensor]:
+    """Several independent pairwise contractions (_PairJob) with one launch per
+    phase over all of them: the operand permutations, the grouped block GEMM
+    (tnx_batch_pairwise; one launch per A layout) and the output
+    permutations.  Each result equals contract_pair's on the same job."""
+    lib = _lib.load()
+    for sel, attr in ((lambda j: j.a_axes is not None, "A"), (lambda j: j.b_axes is not None, "B")):
+        todo = [j for j in jobs if sel(j)]
+        if todo:
+            axes = [j.a_axes if attr == "A" else j.b_axes for j in todo]
+            moved = _batched_transposes([getattr(j, attr) for j in todo], axes)
+            for j, T in zip(todo, moved):
+                setattr(j, attr, T)
+                if attr == "A":
+                    j.a_axes = None
+                else:
+                    j.b_axes = None
+    planned = [plans.pairwise_plan(j.A, j.B, j.a_kfirst, 1, j.nc, j.nat_indices, j.mode, j.target, j.sign_pairs)
+               for j in jobs]
+    starts = np.concatenate([[0], np.cumsum([p[4] for p in planned])]).astype(np.int64)
+    dev = jobs[0].A.data.device
+    buf = torch.empty(int(starts[-1]), dtype=torch.float64, device=dev)
+    res = [SymmetricTensor._wrap(j.nat_indices, p[1], p[2], p[3], buf[starts[k]:starts[k + 1]])
+           for k, (j, p) in enumerate(zip(jobs, planned))]
+    for kfirst in (0, 1):
+        ks = [k for k, j in enumerate(jobs) if j.a_kfirst == kfirst and planned[k][4] > 0]
+        if not ks:
+            continue
+        n = len(ks)
+        P = (_lib.C.c_void_p * n)(*[planned[k][0].ptr for k in ks])
+        Aa = (_lib.C.c_void_p * n)(*[_lib.dptr(jobs[k].A.data.detach()) for k in ks])
+        Bb = (_lib.C.c_void_p * n)(*[_lib.dptr(jobs[k].B.data.detach()) for k in ks])
+        Cc = (_lib.C.c_void_p * n)(*[_lib.dptr(res[k].data) for k in ks])
+        batch = _lib.C.c_void_p()
+        _lib.check(lib.tnx_batch_pairwise(P, n, Aa, Bb, Cc, _lib.stream_handle(), _lib.C.byref(batch)))
+        bplan = _lib.Plan(batch)
+        if WORK["enabled"]:
+            WORK["flops"] += bplan.stats()["flops"]
+            WORK["launches"] += 1
+        _lib.check(lib.tnx_batch_execute(bplan.ptr, _lib.COMPUTE[_state["compute"]], _lib.stream_handle()))
+    fin = [k for k, j in enumerate(jobs) if j.out_axes is not None]
+    if fin:
+        moved = _batched_transposes([res[k] for k in fin], [jobs[k].out_axes for k in fin])
+        for k, T in zip(fin, moved):
+            res[k] = T
+    return res
+
+
+def contract_many(operands, output_labels: Sequence[Label] | None = None) -> list[Tensor]:
+    """``[contract(A, B, output_labels=output_labels) for A, B in operands]``
+    with every instance in the same launches (one grouped block-GEMM launch
+    for all instances' block pairs, one launch per operand / output
+    permutation phase): the many-instance HOTRG step (hotrg.py:109-215 for
+    every temperature at once).  Instances that are not plain
+    SymmetricTensor pairs (dense, fermionic, differentiated, full
+    contractions) are contracted one by one by contract()."""
+    out = [None] * len(operands)
+    jobs, where = [], []
+    for k, (A, B) in enumerate(operands):
+        plain = (isinstance(A, SymmetricTensor) and isinstance(B, SymmetricTensor) and not _is_graded(A)
+                 and not (torch.is_grad_enabled() and (A.data.requires_grad or B.data.requires_grad)))
+        if plain:
+            subscripts, out_idx = _labels_to_subscripts((A, B), output_labels)
+            lhs, so = subscripts.split("->")
+            sa, sb = lhs.split(",")
+            plain = bool(so) and len(set(sa)) == len(sa) and len(set(sb)) == len(sb)
+        if not plain:
+            out[k] = contract(A, B, output_labels=output_labels)
+            continue
+        total = 0
+        for t in (A, B):
+            u = _uniform_target(t)
+            if u is not None:
+                total += u
+        jobs.append(_PairJob(A, B, sa, sb, so, tuple(out_idx), _lib.MODE_PAIRWISE_FILTERED,
+                             total if total != 0 else None))
+        where.append(k)
+    if jobs:
+        for k, C in zip(where, contract_pairs_batched(jobs)):
+            out[k] = C
+    return out
 
 
 def _full_contraction(A, B, sa, sb, sign_chars=()):

@@ -214,7 +214,7 @@ void enumerate_keys(const tnx_leg *legs, int rank, int sym_n, bool all_tuples, b
                     hipStream_t s, KeySet &out);
 
 // ---------------------------------------------------------------- plans
-enum class PlanKind { Pairwise, Transpose, Matvec, EnvLeft, EnvRight };
+enum class PlanKind { Pairwise, Transpose, Matvec, EnvLeft, EnvRight, BatchPairwise, BatchTranspose };
 
 struct tnx_plan_base {
   virtual ~tnx_plan_base() = default;

@@ -726,8 +726,26 @@ constexpr int PERM_RUN_MIN = 16;  // tile mode: shortest run target tried
 
 struct TransposePlan : public tnx_plan_s {
   DevBuf<PDesc> desc;
+  std::vector<PDesc> h_desc;  // host copy (batched execution merges these)
   int32_t n_desc = 0;
   int64_t n_wg = 0;
+};
+
+// Several instances' plans bound to their operand buffers and merged into
+// one launch: the grouped GEMM of many pairwise plans (pair and output
+// offsets rebased onto the lowest operand address of each kind), or the leg
+// permutation of many transpose plans.
+struct BatchPairwisePlan : public tnx_plan_s {
+  GemmProgram prog;
+  const double *A = nullptr, *B = nullptr;
+  double *C = nullptr;
+};
+struct BatchTransposePlan : public tnx_plan_s {
+  DevBuf<PDesc> desc;
+  int32_t n_desc = 0;
+  int64_t n_wg = 0;
+  const double *in = nullptr;
+  double *out = nullptr;
 };
 
 static void build_pairwise(PairwisePlan &P, const tnx_blocks *a, const tnx_blocks *b, int a_kfirst, int b_kfirst,
@@ -1345,6 +1363,7 @@ static void build_transpose(TransposePlan &P, const tnx_blocks *a, const tnx_leg
   P.out_total = off;
   P.bytes = 16.0 * off;
   P.desc.upload(desc.data(), nb, s);
+  P.h_desc = desc;
   P.n_desc = nb;
   P.n_wg = wg;
   P.n_tasks = (int32_t)wg;
@@ -1573,6 +1592,158 @@ int tnx_plan_transpose_graded(const tnx_blocks *a, const tnx_leg *legs, const in
     std::unique_ptr<TransposePlan> P(new TransposePlan());
     build_transpose(*P, a, legs, perm, (hipStream_t)stream, graded != 0);
     *plan = P.release();
+  });
+}
+
+// Lowest address among the instances' buffers of one kind, and each
+// buffer's element offset from it (buffers 8-byte aligned).
+static uintptr_t lowest(const void *const *ptr, const std::vector<int> &live) {
+  uintptr_t lo = UINTPTR_MAX;
+  for (int i : live) {
+    TNX_REQUIRE(ptr[i], TNX_ERR_ARG, "null buffer in a batch");
+    TNX_REQUIRE(((uintptr_t)ptr[i] & 7) == 0, TNX_ERR_ARG, "batch buffers must be 8-byte aligned");
+    lo = std::min(lo, (uintptr_t)ptr[i]);
+  }
+  return lo;
+}
+
+// The output ranges [ptr, ptr + 8 * len) of the instances must not overlap.
+static void check_disjoint(const void *const *ptr, const std::vector<int64_t> &len, const std::vector<int> &live) {
+  std::vector<std::pair<uintptr_t, uintptr_t>> r;
+  for (int i : live)
+    if (len[i] > 0) r.push_back({(uintptr_t)ptr[i], (uintptr_t)ptr[i] + 8 * (uintptr_t)len[i]});
+  std::sort(r.begin(), r.end());
+  for (size_t k = 1; k < r.size(); ++k)
+    TNX_REQUIRE(r[k].first >= r[k - 1].second, TNX_ERR_ARG, "the outputs of a batch overlap");
+}
+
+int tnx_batch_pairwise(const tnx_plan *const *plans, int32_t n, const double *const *a, const double *const *b,
+                       double *const *c, void *stream, tnx_plan **batch) {
+  return guarded([&] {
+    TNX_REQUIRE(batch && plans && a && b && c && n >= 1, TNX_ERR_ARG, "null argument or empty batch");
+    const hipStream_t s = (hipStream_t)stream;
+    std::vector<int> live;
+    std::vector<int64_t> out_len(n, 0);
+    int a_lay = -1;
+    for (int i = 0; i < n; ++i) {
+      TNX_REQUIRE(plans[i] && plans[i]->kind == PlanKind::Pairwise, TNX_ERR_ARG, "batch entry is not a pairwise plan");
+      const PairwisePlan &P = static_cast<const PairwisePlan &>(*plans[i]);
+      out_len[i] = P.out_total;
+      if (P.prog.n_outs == 0 || P.prog.n_tasks == 0) continue;
+      TNX_REQUIRE(a_lay < 0 || a_lay == P.prog.a_lay, TNX_ERR_ARG, "the plans of a batch mix A operand layouts");
+      a_lay = P.prog.a_lay;
+      live.push_back(i);
+    }
+    auto B = std::make_unique<BatchPairwisePlan>();
+    B->kind = PlanKind::BatchPairwise;
+    if (live.empty()) {
+      *batch = B.release();
+      return;
+    }
+    check_disjoint((const void *const *)c, out_len, live);
+    const uintptr_t amin = lowest((const void *const *)a, live), bmin = lowest((const void *const *)b, live),
+                    cmin = lowest((const void *const *)c, live);
+    GemmProgram &G = B->prog;
+    G.a_lay = a_lay;
+    G.b_lay = LAY_KN;
+    std::vector<GemmPair> pairs;
+    std::vector<GemmOut> outs;
+    for (int i : live) {
+      const GemmProgram &g = static_cast<const PairwisePlan &>(*plans[i]).prog;
+      const int64_t da = (int64_t)(((uintptr_t)a[i] - amin) / 8), db = (int64_t)(((uintptr_t)b[i] - bmin) / 8),
+                    dc = (int64_t)(((uintptr_t)c[i] - cmin) / 8);
+      const int32_t p0 = (int32_t)pairs.size();
+      for (GemmPair p : g.h_pairs) {
+        p.a_off += da;
+        p.b_off += db;
+        pairs.push_back(p);
+      }
+      for (GemmOut o : g.h_outs) {
+        o.c_off += dc;
+        o.pair_begin += p0;
+        o.pair_end += p0;
+        outs.push_back(o);
+      }
+      TNX_REQUIRE(pairs.size() < (size_t(1) << 31) && outs.size() < (size_t(1) << 31), TNX_ERR_CAPACITY,
+                  "batch too large");
+    }
+    G.n_pairs = (int32_t)pairs.size();
+    G.n_outs = (int32_t)outs.size();
+    G.pairs.upload(pairs.data(), pairs.size(), s);
+    G.outs.upload(outs.data(), outs.size(), s);
+    finalize_program(G, s);
+    B->A = (const double *)amin;
+    B->B = (const double *)bmin;
+    B->C = (double *)cmin;
+    B->flops = G.flops;
+    B->n_pairs = G.n_pairs;
+    B->n_tasks = G.n_tasks;
+    *batch = B.release();
+  });
+}
+
+int tnx_batch_transpose(const tnx_plan *const *plans, int32_t n, const double *const *in, double *const *out,
+                        void *stream, tnx_plan **batch) {
+  return guarded([&] {
+    TNX_REQUIRE(batch && plans && in && out && n >= 1, TNX_ERR_ARG, "null argument or empty batch");
+    std::vector<int> live;
+    std::vector<int64_t> out_len(n, 0);
+    for (int i = 0; i < n; ++i) {
+      TNX_REQUIRE(plans[i] && plans[i]->kind == PlanKind::Transpose, TNX_ERR_ARG,
+                  "batch entry is not a transpose plan");
+      const TransposePlan &P = static_cast<const TransposePlan &>(*plans[i]);
+      out_len[i] = P.out_total;
+      if (P.n_wg > 0) live.push_back(i);
+    }
+    auto B = std::make_unique<BatchTransposePlan>();
+    B->kind = PlanKind::BatchTranspose;
+    if (live.empty()) {
+      *batch = B.release();
+      return;
+    }
+    check_disjoint((const void *const *)out, out_len, live);
+    const uintptr_t imin = lowest((const void *const *)in, live), omin = lowest((const void *const *)out, live);
+    std::vector<PDesc> desc;
+    int64_t wg = 0;
+    for (int i : live) {
+      const TransposePlan &P = static_cast<const TransposePlan &>(*plans[i]);
+      const int64_t di = (int64_t)(((uintptr_t)in[i] - imin) / 8), dout = (int64_t)(((uintptr_t)out[i] - omin) / 8);
+      for (PDesc d : P.h_desc) {
+        d.in_off += di;
+        d.out_off += dout;
+        d.task0 += wg;
+        desc.push_back(d);
+      }
+      wg += P.n_wg;
+    }
+    TNX_REQUIRE(wg < (int64_t(1) << 31) && desc.size() < (size_t(1) << 31), TNX_ERR_CAPACITY, "batch too large");
+    B->desc.upload(desc.data(), desc.size(), (hipStream_t)stream);
+    B->n_desc = (int32_t)desc.size();
+    B->n_wg = wg;
+    B->in = (const double *)imin;
+    B->out = (double *)omin;
+    B->n_tasks = (int32_t)wg;
+    TNX_CHECK_HIP(hipStreamSynchronize((hipStream_t)stream));
+    *batch = B.release();
+  });
+}
+
+int tnx_batch_execute(const tnx_plan *batch, int32_t compute, void *stream) {
+  return guarded([&] {
+    TNX_REQUIRE(batch, TNX_ERR_ARG, "null plan");
+    const hipStream_t s = (hipStream_t)stream;
+    if (batch->kind == PlanKind::BatchPairwise) {
+      const BatchPairwisePlan &B = static_cast<const BatchPairwisePlan &>(*batch);
+      if (B.prog.n_tasks == 0) return;
+      launch_grouped_gemm(B.prog, B.A, B.B, B.C, compute, false, s);
+    } else if (batch->kind == PlanKind::BatchTranspose) {
+      const BatchTransposePlan &B = static_cast<const BatchTransposePlan &>(*batch);
+      if (B.n_wg == 0) return;
+      hipLaunchKernelGGL(permute_kernel, dim3((unsigned)B.n_wg), dim3(256), 0, s, B.desc.p, B.n_desc, B.in, B.out);
+      TNX_CHECK_HIP(hipGetLastError());
+    } else {
+      throw Error{TNX_ERR_ARG, "not a batch plan"};
+    }
   });
 }
 

@@ -152,6 +152,66 @@ def test_contract_needs_transposes_and_swap(gpu):
         np.testing.assert_allclose(C.todense().cpu().numpy(), ref, atol=1e-12)
 
 
+def test_contract_many_equals_one_by_one(gpu):
+    """contract_many (every instance's operand permutations, block GEMMs and
+    output permutations in shared launches) returns, instance by instance,
+    exactly what contract() returns: mixed structures (U(1) and Z2, operands
+    needing transposes on both sides or none, K-first A layouts, requested
+    output orders), a dense pair and a fermionic pair taking the one-by-one
+    path, and an instance with no matching blocks."""
+    from tenax_amd import contract
+    from tenax_amd.contraction import contract_many
+    from tenax_amd.core.tensor import DenseTensor
+
+    rng = np.random.default_rng(11)
+    U1, Z2 = oc.U1(), oc.Zn(2)
+    insts, outs = [], []
+    for k in range(6):
+        sym = U1 if k % 2 == 0 else Z2
+        ch = (lambda n: rng.integers(-2, 3, n)) if sym is U1 else (lambda n: rng.integers(0, 2, n))
+        bi, bj = ch(7 + k), ch(5 + k)
+        A = (oc.Leg(sym, bi, oc.IN, "i"), oc.Leg(sym, ch(4), oc.OUT, "x"), oc.Leg(sym, bj, oc.IN, "j"),
+             oc.Leg(sym, ch(6), oc.OUT, "k"))
+        B = (oc.Leg(sym, bj, oc.OUT, "j"), oc.Leg(sym, ch(3), oc.IN, "y"), oc.Leg(sym, bi, oc.OUT, "i"),
+             oc.Leg(sym, ch(5), oc.IN, "z"))
+        if k == 3:  # A already [contracted..., free...]: K-first layout, no transpose
+            A = (A[0], A[2], A[1], A[3])
+        insts.append((from_oracle(oc.SymT.random_normal(A, (20 + k,))),
+                      from_oracle(oc.SymT.random_normal(B, (30 + k,)))))
+    # no matching charges at all: an empty result
+    e1 = (oc.Leg(U1, np.array([5, 6]), oc.IN, "i"), oc.Leg(U1, np.array([0]), oc.OUT, "x"))
+    e2 = (oc.Leg(U1, np.array([-1, 1]), oc.OUT, "i"), oc.Leg(U1, np.array([0]), oc.IN, "y"))
+    insts.append((from_oracle(oc.SymT.random_normal(e1, (1,), target=None)),
+                  from_oracle(oc.SymT.random_normal(e2, (2,), target=None))))
+    for order in [None, ("z", "x", "y", "k")]:
+        got = contract_many(insts[:-1], output_labels=order) + contract_many(insts[-1:])
+        for (A, B), C in zip(insts[:-1], got[:-1]):
+            ref = contract(A, B, output_labels=order)
+            assert C.labels() == ref.labels()
+            assert np.array_equal(C.keys, ref.keys) and np.array_equal(C.shapes, ref.shapes)
+            assert torch.equal(C.data, ref.data)
+        ref = contract(*insts[-1])
+        assert got[-1].n_blocks == ref.n_blocks == 0
+    # a dense pair and a fermionic pair go through contract() one by one
+    d1 = torch.tensor(rng.standard_normal((3, 4)), device="cuda")
+    d2 = torch.tensor(rng.standard_normal((4, 5)), device="cuda")
+    from tenax_amd.core.index import FlowDirection, TensorIndex
+    from tenax_amd.core.symmetry import U1Symmetry
+
+    ix = lambda n, l: TensorIndex(U1Symmetry(), np.zeros(n, np.int32), FlowDirection.IN, l)
+    D1, D2 = DenseTensor(d1, (ix(3, "a"), ix(4, "b"))), DenseTensor(d2, (ix(4, "b"), ix(5, "c")))
+    FP = oc.FermionParity()
+    bf = rng.integers(0, 2, 6)
+    F1 = (oc.Leg(FP, rng.integers(0, 2, 4), oc.IN, "p"), oc.Leg(FP, bf, oc.OUT, "q"))
+    F2 = (oc.Leg(FP, bf, oc.IN, "q"), oc.Leg(FP, rng.integers(0, 2, 3), oc.OUT, "r"))
+    fA = from_oracle(oc.SymT.random_normal(F1, (40,), target=None))
+    fB = from_oracle(oc.SymT.random_normal(F2, (41,), target=None))
+    got = contract_many([(D1, D2), insts[0], (fA, fB)])
+    assert torch.allclose(got[0].todense(), d1 @ d2, rtol=0, atol=1e-13)
+    assert torch.equal(got[1].data, contract(*insts[0]).data)
+    assert torch.equal(got[2].data, contract(fA, fB).data)
+
+
 @pytest.mark.parametrize("n", [2, 3])
 def test_contract_zn(gpu, n):
     from tenax_amd import contract
