@@ -435,16 +435,40 @@ def contract_many(operands, output_labels: Sequence[Label] | None = None) -> lis
 
 
 def _full_contraction(A, B, sa, sb, sign_chars=()):
+    """All legs contracted: the sum over common keys of <A block, B block>
+    (with the fermionic exchange sign per block).  One dot over the packed
+    buffers when both operands hold the same blocks and no sign is negative;
+    otherwise the common blocks (A's signed) are gathered by one batched copy
+    each and dotted once."""
     Bp = B if sb == sa else _transpose_raw(B, tuple(sb.index(c) for c in sa), graded=False)
     ka, kb = A.key_index(), Bp.key_index()
-    tot = torch.zeros((), dtype=torch.float64, device=A.data.device)
-    for k, i in ka.items():
-        j = kb.get(k)
-        if j is not None:
-            q = dict(zip(sa, k))
-            neg = sum((q[x] & 1) * (q[y] & 1) for x, y in sign_chars) % 2
+    common = [(i, kb[k], k) for k, i in ka.items() if k in kb]
+    signs = [(-1.0 if sum((dict(zip(sa, k))[x] & 1) * (dict(zip(sa, k))[y] & 1) for x, y in sign_chars) % 2
+              else 1.0) for _, _, k in common]
+    dev = A.data.device
+    same = (len(common) == A.n_blocks == Bp.n_blocks and np.array_equal(A.shapes, Bp.shapes)
+            and np.array_equal(A.offsets, Bp.offsets) and all(i == j for i, j, _ in common))
+    if not common:
+        tot = torch.zeros((), dtype=torch.float64, device=dev)
+    elif same and all(v > 0 for v in signs):
+        tot = torch.dot(A.data, Bp.data)
+    elif torch.is_grad_enabled() and (A.data.requires_grad or Bp.data.requires_grad):
+        # differentiable form (the batched copy below is not): block views
+        tot = torch.zeros((), dtype=torch.float64, device=dev)
+        for (i, j, _), sg in zip(common, signs):
             d = torch.dot(A.block(i).reshape(-1), Bp.block(j).reshape(-1))
-            tot = tot - d if neg else tot + d
+            tot = tot - d if sg < 0 else tot + d
+    else:
+        sizes = [int(np.prod(A.shapes[i])) for i, _, _ in common]
+        starts = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        ga = torch.empty(int(starts[-1]), dtype=torch.float64, device=dev)
+        gb = torch.empty_like(ga)
+        ad, bd = A.data.detach(), Bp.data.detach()
+        _lib.copy2d([(ad, int(A.offsets[i]), n, ga, int(o), n, 1, n, sg)
+                     for (i, _, _), n, o, sg in zip(common, sizes, starts, signs)], dev)
+        _lib.copy2d([(bd, int(Bp.offsets[j]), n, gb, int(o), n, 1, n, 1.0)
+                     for (_, j, _), n, o in zip(common, sizes, starts)], dev)
+        tot = torch.dot(ga, gb)
     return SymmetricTensor._wrap((), np.zeros((1, 0), np.int32), np.zeros((1, 0), np.int32),
                                  np.zeros(1, np.int64), tot.reshape(1))
 

@@ -212,6 +212,45 @@ def test_contract_many_equals_one_by_one(gpu):
     assert torch.equal(got[2].data, contract(fA, fB).data)
 
 
+def test_full_contraction_matches_dense(gpu):
+    """Every leg contracted (a scalar): the same block structure on both sides
+    (one packed dot), different block sets and leg orders (common blocks
+    gathered), a fermionic pair against the oracle's signed contraction, and
+    the gradient of the scalar."""
+    from tenax_amd import contract
+
+    U1 = oc.U1()
+    rng = np.random.default_rng(21)
+    a, b, c = rng.integers(-2, 3, 6), rng.integers(-1, 2, 5), rng.integers(-2, 3, 7)
+    L1 = (oc.Leg(U1, a, oc.IN, "i"), oc.Leg(U1, b, oc.OUT, "j"), oc.Leg(U1, c, oc.OUT, "k"))
+    L2 = (oc.Leg(U1, a, oc.OUT, "i"), oc.Leg(U1, b, oc.IN, "j"), oc.Leg(U1, c, oc.IN, "k"))
+    oA = oc.SymT.random_normal(L1, (3,))
+    oB = oc.SymT.random_normal(L2, (4,))
+    ref = float(np.einsum("ijk,ijk->", oA.todense(), oB.todense()))
+    A, B = from_oracle(oA), from_oracle(oB)
+    assert abs(float(contract(A, B).data[0]) - ref) <= 1e-12 * max(1.0, abs(ref))
+    # B with its legs permuted and two of its blocks absent (common blocks gathered)
+    oC = oc.SymT.random_normal((L2[2], L2[0], L2[1]), (5,))
+    for key in sorted(oC.blocks)[1:3]:
+        del oC.blocks[key]
+    refc = float(np.einsum("ijk,kij->", oA.todense(), oC.todense()))
+    assert abs(float(contract(A, from_oracle(oC)).data[0]) - refc) <= 1e-12 * max(1.0, abs(refc))
+    # gradient of <A, B> with respect to A is B (same structure)
+    Ag = A._with_data(A.data.clone().requires_grad_(True))
+    contract(Ag, B).data.sum().backward()
+    assert torch.equal(Ag.data.grad, B.data)
+    # fermionic: the oracle's signed full contraction
+    FP = oc.FermionParity()
+    p, q = rng.integers(0, 2, 4), rng.integers(0, 2, 5)
+    F1 = (oc.Leg(FP, p, oc.IN, "p"), oc.Leg(FP, q, oc.OUT, "q"))
+    F2 = (oc.Leg(FP, q, oc.IN, "q"), oc.Leg(FP, p, oc.OUT, "p"))
+    oF1, oF2 = oc.SymT.random_normal(F1, (6,)), oc.SymT.random_normal(F2, (7,))
+    oS, _ = ocn.contract_symmetric([oF1, oF2], "pq,qp->")
+    want = sum(float(np.sum(v)) for v in oS.blocks.values())
+    got = float(contract(from_oracle(oF1), from_oracle(oF2)).data[0])
+    assert abs(got - want) <= 1e-12 * max(1.0, abs(want))
+
+
 @pytest.mark.parametrize("n", [2, 3])
 def test_contract_zn(gpu, n):
     from tenax_amd import contract
