@@ -276,8 +276,10 @@ def segment_comparison(pd, mps, mpo, chi, reorth, n_bonds=20):
     """Full left-to-right bond updates (two-site Lanczos, truncated SVD, left
     environment update; dmrg.py:184-214) over the n_bonds bonds around the
     centre, from the same MPS and environments, on the fused/grouped path and
-    on the plain per-block GPU path (one launch per block product, Lanczos
-    scalars through the host, one SVD call per sector)."""
+    on the same path with every Lanczos solve replayed from a captured graph
+    re-recorded per bond (lanczos_graph), and on the plain per-block GPU path
+    (one launch per block product, Lanczos scalars through the host, one SVD
+    call per sector)."""
     import torch
 
     from tenax_amd import DMRGConfig
@@ -288,9 +290,9 @@ def segment_comparison(pd, mps, mpo, chi, reorth, n_bonds=20):
     right = pd._build_right_environments_list(mps, mpo, L, ops)
     c0 = L // 2 - n_bonds // 2
     out = {"bonds": [c0, c0 + n_bonds - 1]}
-    for name, per_block in (("fused", False), ("per_block", True)):
+    for name, per_block, graph in (("fused_eager", False, False), ("fused", False, True), ("per_block", True, False)):
         cfg = DMRGConfig(max_bond_dim=chi, num_sweeps=1, lanczos_max_iter=50, lanczos_reorth=reorth,
-                         per_block=per_block)
+                         per_block=per_block, lanczos_graph=graph)
         update = pd._per_block_two_site if per_block else ops.two_site_update
         sites, left = list(mps), list(left0)
         pd._ACTIVE_CONFIG["cfg"] = cfg
@@ -306,6 +308,7 @@ def segment_comparison(pd, mps, mpo, chi, reorth, n_bonds=20):
             out[name] = {"seconds": time.time() - t0, "last_energy": float(e)}
         finally:
             pd._ACTIVE_CONFIG.pop("cfg", None)
+            pd.release_graphs()
         log(f"segment {name}: {out[name]}")
     out["speedup_fused"] = out["per_block"]["seconds"] / out["fused"]["seconds"]
     return out

@@ -119,6 +119,11 @@ int tnx_lanczos_noalloc(const tnx_plan *matvec_plan, const double *L, const doub
  *   create   records the solve for these operand and theta pointers;
  *   update   re-records it for new pointers (or after the plan's tiling
  *            changed) and updates the executable graph in place;
+ *   rebind   re-records it for another matvec plan (the next two-site
+ *            update of a sweep) and its buffers, growing the workspace when
+ *            the new problem is larger; the executable graph is updated in
+ *            place when the new recording has the same launch topology, and
+ *            instantiated anew otherwise;
  *   launch   replays it (TNX_ERR_ARG if tnx_plan_set_tiling changed the plan
  *            since the last create/update);
  *   result   synchronises the stream and returns energy and step count
@@ -138,6 +143,9 @@ int tnx_lanczos_graph_create(const tnx_plan *matvec_plan, const double *L, const
                              tnx_graph **graph);
 int tnx_lanczos_graph_update(tnx_graph *graph, const double *L, const double *Wl, const double *Wr,
                              const double *R, double *theta, void *stream);
+int tnx_lanczos_graph_rebind(tnx_graph *graph, const tnx_plan *matvec_plan, const double *L,
+                             const double *Wl, const double *Wr, const double *R, double *theta,
+                             void *stream);
 int tnx_graph_launch(tnx_graph *graph, void *stream);
 int tnx_lanczos_graph_result(tnx_graph *graph, void *stream, double *energy, int32_t *n_steps);
 int tnx_graph_free(tnx_graph *graph);

@@ -115,6 +115,7 @@ SIGNATURES = {
     "tnx_lanczos_graph_create": (C.c_int, [_p, _p, _p, _p, _p, _p, _i32, C.c_double, _i32, _i32, _p,
                                            C.POINTER(_p)]),
     "tnx_lanczos_graph_update": (C.c_int, [_p, _p, _p, _p, _p, _p, _p]),
+    "tnx_lanczos_graph_rebind": (C.c_int, [_p, _p, _p, _p, _p, _p, _p, _p]),
     "tnx_graph_launch": (C.c_int, [_p, _p]),
     "tnx_lanczos_graph_result": (C.c_int, [_p, _p, _PD, _PI32]),
     "tnx_graph_free": (C.c_int, [_p]),

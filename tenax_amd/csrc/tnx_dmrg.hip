@@ -2026,6 +2026,7 @@ struct tnx_graph_s {
   uint64_t gen = 0;  // plan generation the captured work lists belong to
   int max_iter = 0, reorth = 0, compute = 0;
   double tol = 0.0;
+  int64_t reinstantiated = 0;  // rebinds whose new recording could not update the executable graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   tnx::LanczosWorkspace ws;  // the graph's own Krylov basis and scalars
@@ -2115,6 +2116,41 @@ int tnx_lanczos_graph_update(tnx_graph *graph, const double *L, const double *Wl
     (void)hipGraphDestroy(G.graph);
     G.graph = ng;
     G.gen = G.plan->generation;
+  });
+}
+
+int tnx_lanczos_graph_rebind(tnx_graph *graph, const tnx_plan *matvec_plan, const double *L, const double *Wl,
+                             const double *Wr, const double *R, double *theta, void *stream) {
+  return guarded([&] {
+    TNX_REQUIRE(graph, TNX_ERR_ARG, "null graph");
+    TNX_REQUIRE(matvec_plan && matvec_plan->kind == PlanKind::Matvec, TNX_ERR_ARG, "not a matvec plan");
+    tnx_graph_s &G = *graph;
+    const FusedPlan &P = static_cast<const FusedPlan &>(*matvec_plan);
+    TNX_CHECK_HIP(hipStreamSynchronize((hipStream_t)stream));  // no replay of the old graph in flight
+    if (&P != G.plan) {
+      P.live_graphs.fetch_add(1);
+      G.plan->live_graphs.fetch_sub(1);
+      G.plan = &P;
+    }
+    lanczos_reserve(G.ws, P.in_total, G.max_iter);  // grows the Krylov basis when the new problem is larger
+    hipGraph_t ng = capture_lanczos(G, L, Wl, Wr, R, theta);
+    hipGraphNode_t err_node = nullptr;
+    hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+    const hipError_t e = hipGraphExecUpdate(G.exec, ng, &err_node, &res);
+    if (e != hipSuccess || res != hipGraphExecUpdateSuccess) {
+      (void)hipGetLastError();
+      (void)hipGraphExecDestroy(G.exec);
+      G.exec = nullptr;
+      const hipError_t ei = hipGraphInstantiate(&G.exec, ng, nullptr, nullptr, 0);
+      if (ei != hipSuccess) {
+        (void)hipGraphDestroy(ng);
+        TNX_CHECK_HIP(ei);
+      }
+      G.reinstantiated += 1;
+    }
+    (void)hipGraphDestroy(G.graph);
+    G.graph = ng;
+    G.gen = P.generation;
   });
 }
 

@@ -289,6 +289,36 @@ def test_dmrg_matches_oracle_trajectory(gpu, reorth, tol_first):
         assert sorted(tuple(int(q) for q in k) for k in t.keys) == sorted(o.blocks)
 
 
+@pytest.mark.parametrize("reorth", [False, True])
+def test_dmrg_with_captured_lanczos_equals_eager(gpu, reorth):
+    """DMRGConfig(lanczos_graph=True): every two-site update's Lanczos solve is
+    a replay of one captured graph rebound to the update's problem (bond
+    dimensions growing along the chain, so the workspace grows and the launch
+    topology changes); energies and the final MPS are bitwise those of the
+    eager solves."""
+    import torch
+
+    import importlib
+
+    from tenax_amd import DMRGConfig, build_mpo_heisenberg, build_random_symmetric_mps, dmrg
+
+    pd = importlib.import_module("tenax_amd.algorithms.dmrg")
+    L = 10
+    mpo = build_mpo_heisenberg(L)
+    runs = []
+    for graph in (False, True):
+        mps = build_random_symmetric_mps(L, bond_dim=4, seed=5)
+        cfg = DMRGConfig(max_bond_dim=24, num_sweeps=3, lanczos_max_iter=30, lanczos_reorth=reorth,
+                         lanczos_graph=graph, convergence_tol=0.0)
+        runs.append(dmrg(mpo, mps, cfg))
+    assert not pd._GRAPHS  # released at the end of the run
+    eager, graph = runs
+    assert eager.energies_per_sweep == graph.energies_per_sweep
+    for i in range(L):
+        a, b = eager.mps.get_tensor(i), graph.mps.get_tensor(i)
+        assert np.array_equal(a.keys, b.keys) and torch.equal(a.data, b.data)
+
+
 def test_dmrg_rejects_mixed_and_dense(gpu):
     from tenax_amd import DMRGConfig, TensorNetwork, build_mpo_heisenberg, build_random_symmetric_mps, dmrg
     from tenax_amd.core.tensor import DenseTensor
