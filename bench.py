@@ -590,7 +590,8 @@ def main():
     ap.add_argument("--no-reorth", action="store_true", help="C3 with the reference's plain Lanczos")
     ap.add_argument("--workload", choices=("dmrg", "hotrg", "ctm"), default="dmrg")
     ap.add_argument("--no-c4c5", dest="c4c5", action="store_false",
-                    help="leave the HOTRG (c5) and CTMRG (c4) summary keys out of the DMRG line")
+                    help="leave the chi=1024 contraction (c2), HOTRG (c5) and CTMRG (c4) summary keys out of the "
+                         "DMRG line")
     ap.add_argument("--matvec-only", action="store_true", help="fused matvec and stage timing only (profiling)")
     ap.add_argument("--calibrate-copy", action="store_true", help="run a 512 MiB device copy (counter calibration)")
     ap.add_argument("--pin-tiling", action="append", default=[], metavar="STAGE:SHAPE,KCHUNK,XCD",
@@ -751,6 +752,8 @@ def main():
         result["c3"]["centre_bond"] = par
         log(f"C3 centre bond: {par}")
         del cprob
+        result["c3"]["centre_bond_reduced_precision"] = c3_reduced_precision(pd, c3_mps, c3_mpo, not args.no_reorth)
+        log(f"C3 centre bond reduced precision: {result['c3']['centre_bond_reduced_precision']}")
         cmp = compare_updates(pd, c3_mps, c3_mpo, args.chi, not args.no_reorth)
         result["two_site_updates_centre_s"] = cmp
         log(f"C3 centre updates fused vs per-block: {cmp}")
@@ -783,9 +786,83 @@ def main():
         result["parity"] = run_c1_parity()
         log(f"parity {result['parity']}")
     if args.c4c5:
+        result["c2"] = c2_summary(args)
         result["c5"] = c5_summary(args)
         result["c4"] = c4_summary(args)
     print(json.dumps(result), flush=True)
+
+
+def c2_summary(args, reps=200):
+    """The single block-sparse contraction at chi=1024 (rank-4 U(1) tensors,
+    21-sector Gaussian multiplicities, N(0,1) entries from seed 1, contracted
+    over r): structure, grouped-GEMM time per launch (HIP events over
+    repeated launches of the cached plan) and GFLOP/s in FP64, FP32 and
+    split FP32, with the reduced-precision errors against the FP64 result."""
+    import torch
+
+    from tenax_amd import SymmetricTensor, contract
+    from tenax_amd.contraction import contractor, plans
+    from tenax_amd.core.index import FlowDirection, TensorIndex
+    from tenax_amd.core.symmetry import U1Symmetry
+    from tenax_amd.workloads import PHYS_CHARGES, gaussian_bond_charges
+
+    U1 = U1Symmetry()
+    bond = gaussian_bond_charges(1024)
+    IN, OUT = FlowDirection.IN, FlowDirection.OUT
+    ia = (TensorIndex(U1, bond, IN, "l"), TensorIndex(U1, PHYS_CHARGES, IN, "p"),
+          TensorIndex(U1, PHYS_CHARGES, IN, "pp"), TensorIndex(U1, bond, OUT, "r"))
+    ib = (TensorIndex(U1, bond, IN, "r"), TensorIndex(U1, PHYS_CHARGES, IN, "q"),
+          TensorIndex(U1, PHYS_CHARGES, IN, "qq"), TensorIndex(U1, bond, OUT, "s"))
+    A = SymmetricTensor.random_normal(ia, (1, 0))
+    B = SymmetricTensor.random_normal(ib, (1, 1))
+    C = contract(A, B)
+    plan = plans.pairwise_plan(A, B, 0, 1, 1, C.indices, 0, None)[0]
+    flops = plan.stats()["flops"]
+    out = {"workload": "A[l,p,p',r] . B[r,q,q',s], U(1), chi=1024 (21 sectors), N(0,1) seed 1",
+           "a_blocks": int(A.n_blocks), "a_elems": int(A.data.numel()), "out_blocks": int(C.n_blocks),
+           "out_elems": int(C.data.numel()), "block_pairs": plan.stats()["n_pairs"], "flops": flops}
+    ref = C.data.clone()
+    for mode in ("f64", "f32", "split_f32"):
+        with contractor.compute_mode(mode):
+            y = torch.empty_like(ref)
+            contractor._execute_pair(plan, A, B, y)
+            ms = time_events(lambda: contractor._execute_pair(plan, A, B, y), reps)
+        out[mode] = {"us": ms * 1e3, "gflops": flops / (ms * 1e-3) / 1e9,
+                     "max_rel_err_vs_f64": float((y - ref).abs().max() / ref.abs().max())}
+    out["f64"]["frac_of_fp64_mfma_peak"] = out["f64"]["gflops"] / 1e3 / PEAK_FP64_MFMA_TFLOPS
+    log(f"c2 chi=1024: {out}")
+    return out
+
+
+def c3_reduced_precision(pd, mps, mpo, reorth):
+    """The C3 centre-bond two-site Lanczos with the matvec GEMMs in FP32 MFMA
+    and in split FP32 (FP64 Lanczos recurrences, re-orthogonalisation and
+    tridiagonal solve), energies against the FP64 solve on the same input."""
+    import torch
+
+    c, left, right = centre_envs(pd, mps, mpo)
+    theta, _, _ = pd._theta_padded(mps[c], mps[c + 1])
+    out = {}
+    x0 = None
+    for mode in ("f64", "f32", "split_f32"):
+        prob = pd.TwoSiteProblem(theta, left, mpo[c], mpo[c + 1], right)
+        if x0 is None:
+            x0 = prob.x.clone()
+        prob.x.copy_(x0)
+        torch.cuda.synchronize()
+        t0 = time.time()
+        e, n = prob.lanczos(50, 1e-12, reorth, _lib_compute(mode))
+        torch.cuda.synchronize()
+        out[mode] = {"energy": e, "steps": n, "seconds": time.time() - t0}
+    for mode in ("f32", "split_f32"):
+        out[mode]["abs_dE_vs_f64"] = abs(out[mode]["energy"] - out["f64"]["energy"])
+    return out
+
+
+def _lib_compute(mode):
+    from tenax_amd import _lib
+
+    return _lib.COMPUTE[mode]
 
 
 def c5_summary(args):

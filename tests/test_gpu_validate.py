@@ -54,7 +54,10 @@ def test_random_pairwise_plans_validate(gpu, seed):
     B = tuple(l.flip_flow() for l in shared) + (leg("c", oc.IN), leg("d", oc.OUT))
     oA = oc.SymT.random_normal(A, (seed, 1))
     oB = oc.SymT.random_normal(B, (seed, 2))
-    before = {id(p) for p in plans.live_pairwise_plans()}
+    # the list keeps the earlier plans alive, so that a plan the cache evicts
+    # meanwhile cannot hand its id to a new one
+    kept = plans.live_pairwise_plans()
+    before = {id(p) for p in kept}
     C = contract(from_oracle(oA), from_oracle(oB))  # built with validation on
     assert C.n_blocks > 0
     n = 0
@@ -82,7 +85,8 @@ def test_environment_and_matvec_plans_of_the_faulted_state_validate(gpu):
     P = [from_oracle(t) for t in mps]
     W = [from_oracle(t) for t in mpo]
     ops = pd._symmetric_ops()
-    before = {id(p) for p in plans.live_env_plans()}
+    kept = plans.live_env_plans()  # alive: their ids cannot be reused
+    before = {id(p) for p in kept}
     left = pd._build_left_environments_list(P, W, 8, ops)
     right = pd._build_right_environments_list(P, W, 8, ops)
     new = [p for p in plans.live_env_plans() if id(p) not in before]
