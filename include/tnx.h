@@ -124,9 +124,12 @@ int tnx_plan_free(tnx_plan *plan);
  * reference dispatches one einsum per block pair).  stage: 0 for a pairwise
  * plan; 1 (L.theta) or 3 (.R) for a matvec / environment plan.
  *   shape  0 = 64x64, 1 = 128x64, 2 = 64x128, 3 = 128x128 output tiles per workgroup
- *   kchunk K tiles (of 16) per task: 0 = whole K, < 0 = chosen from the work per
- *          workgroup slot; outputs split in K are summed by a second launch in a
- *          fixed order (deterministic)
+ *   kchunk K tiles (of 16) per task: 0 = whole K, -2 = balanced two-wave split
+ *          (tiles whole up to the mean work of one workgroup slot, longer ones
+ *          cut into a head of that length, dispatched with the whole tiles,
+ *          and short tail pieces dispatched after them), other values < 0 =
+ *          equal chunks chosen from the work per workgroup slot; outputs split
+ *          in K are summed by a second launch in a fixed order (deterministic)
  *   xcd    1 = place the tiles of one output (or tile row) on one XCD, bins balanced
  * The result stays deterministic for a given tiling; different tilings differ
  * by rounding.  tnx_plan_get_tiling returns the active tiling and task count. */

@@ -22,11 +22,15 @@ import threading
 
 import torch
 
-# (shape, kchunk, xcd): shape 0 = 64x64, 1 = 128x64, 2 = 64x128; kchunk -1 = split K automatically.
-# Near-ties go to the earliest candidate: the XCD-ordered whole-K tiling first
-# (it moves ~30% fewer bytes from HBM than the plain order at equal kernel
-# time, and needs no split-K reduction launch), the default tiling second.
-CANDIDATES = [(0, 0, 1), (0, 0, 0), (0, -1, 1), (0, -1, 0), (1, 0, 1), (2, 0, 1), (1, -1, 1), (2, -1, 1)]
+# (shape, kchunk, xcd): shape 0 = 64x64, 1 = 128x64, 2 = 64x128; kchunk -1 = split K automatically,
+# -2 = the balanced two-wave split (whole tiles and heads of the long ones at
+# once, short tail pieces after).  Near-ties go to the earliest candidate: the
+# XCD-ordered whole-K tiling first (it moves ~30% fewer bytes from HBM than
+# the plain order at equal kernel time, and needs no split-K reduction
+# launch), the default tiling second, then the balanced split (fewer
+# partial tiles than the plain split).
+CANDIDATES = [(0, 0, 1), (0, 0, 0), (0, -2, 1), (0, -1, 1), (0, -1, 0), (1, 0, 1), (2, 0, 1), (1, -1, 1),
+              (2, -1, 1)]
 MIN_FLOPS = 2e9  # below this a launch is too short for the tiling to matter
 REPS = 2
 TIE = 0.05  # switch away from an earlier candidate only for a clear gain

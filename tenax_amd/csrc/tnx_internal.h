@@ -140,6 +140,8 @@ struct SplitTile {
 //   shape: 0 = 64x64, 1 = 128x64, 2 = 64x128, 3 = 128x128 (rows x cols per workgroup)
 //   kchunk: K tiles per task; 0 = no split, < 0 = chosen from the work per workgroup slot
 //   xcd: order the tasks so that tiles of one output run on one XCD (shared L2)
+// kchunk value selecting the balanced two-wave split-K schedule
+constexpr int32_t KCHUNK_BALANCED = -2;
 struct TileCfg {
   int32_t shape = 0;
   int32_t kchunk = 0;

@@ -509,8 +509,16 @@ void set_tiling(GemmProgram &prog, const TileCfg &cfg, hipStream_t s) {
     total += (double)ntile[o] * (double)std::max<int64_t>(c, 1);
   }
   // K tiles per task: bounded by half the mean work of one resident workgroup
-  int64_t chunk = cfg.kchunk;
-  if (chunk < 0) chunk = std::max<int64_t>(8, (int64_t)std::ceil(total / (2.0 * 256 * wg_per_cu(cfg.shape))));
+  // (kchunk < 0), or the balanced two-wave schedule (kchunk == KCHUNK_BALANCED):
+  // a first wave of whole tiles and heads of the long ones, cut at the mean
+  // work of one workgroup slot (all of it dispatched at once), and a second
+  // wave of short tail pieces that the slots pick up as they free
+  const bool balanced = cfg.kchunk == KCHUNK_BALANCED;
+  const double slots = 256.0 * wg_per_cu(cfg.shape);
+  const int64_t cap = std::max<int64_t>(8, (int64_t)std::ceil(total / slots));
+  const int64_t piece = std::max<int64_t>(4, cap / 8);
+  int64_t chunk = balanced ? 0 : cfg.kchunk;
+  if (chunk < 0) chunk = std::max<int64_t>(8, (int64_t)std::ceil(total / (2.0 * slots)));
   // work units (an output, or a row of tiles of a large output) with their tasks
   struct Unit {
     int64_t cost;
@@ -529,7 +537,7 @@ void set_tiling(GemmProgram &prog, const TileCfg &cfg, hipStream_t s) {
   }
   std::stable_sort(units.begin(), units.end(), [](const Unit &x, const Unit &y) { return x.cost > y.cost; });
   const int nbin = cfg.xcd ? N_XCD : 1;
-  std::vector<std::vector<GemmTask>> bins(nbin);
+  std::vector<std::vector<GemmTask>> bins(nbin), tails(nbin);
   std::vector<int64_t> load(nbin, 0);
   std::vector<SplitTile> red;
   int64_t slot = 0;
@@ -539,6 +547,25 @@ void set_tiling(GemmProgram &prog, const TileCfg &cfg, hipStream_t s) {
     const GemmOut &g = ho[u.out];
     const int tn = (g.N + bn - 1) / bn;
     const int64_t K = kt[u.out];
+    if (balanced) {
+      // head [0, cap) in the first wave, the rest in pieces of `piece` K tiles
+      const int nsplit = K > cap ? 1 + (int)((K - cap + piece - 1) / piece) : 1;
+      for (int a = u.mt0; a < u.mt1; ++a)
+        for (int c = 0; c < tn; ++c) {
+          if (nsplit == 1) {
+            bins[b].push_back(whole_tile(u.out, a * bm, c * bn));
+            continue;
+          }
+          red.push_back(SplitTile{u.out, a * bm, c * bn, (int32_t)slot, nsplit, 0});
+          bins[b].push_back(GemmTask{u.out, a * bm, c * bn, 0, (int32_t)cap, (int32_t)slot, 0, 0});
+          for (int q = 1; q < nsplit; ++q) {
+            const int32_t k0 = (int32_t)(cap + (q - 1) * piece), k1 = (int32_t)std::min<int64_t>(K, cap + q * piece);
+            tails[b].push_back(GemmTask{u.out, a * bm, c * bn, k0, k1, (int32_t)(slot + q), 0, 0});
+          }
+          slot += nsplit;
+        }
+      continue;
+    }
     const int nsplit = (chunk > 0 && K > chunk) ? (int)((K + chunk - 1) / chunk) : 1;
     for (int a = u.mt0; a < u.mt1; ++a)
       for (int c = 0; c < tn; ++c) {
@@ -561,9 +588,15 @@ void set_tiling(GemmProgram &prog, const TileCfg &cfg, hipStream_t s) {
     return std::min<int64_t>(t.kt1, std::max<int64_t>(kt[t.out], 1)) - t.kt0;
   };
   size_t rounds = 0;
-  for (auto &v : bins) {
+  for (int b = 0; b < nbin; ++b) {
+    auto &v = bins[b];
     std::stable_sort(v.begin(), v.end(),
                      [&](const GemmTask &x, const GemmTask &y) { return cost(x, kt) > cost(y, kt); });
+    // balanced schedule: the second wave (tail pieces, longest first) after all of the first
+    auto &w = tails[b];
+    std::stable_sort(w.begin(), w.end(),
+                     [&](const GemmTask &x, const GemmTask &y) { return cost(x, kt) > cost(y, kt); });
+    v.insert(v.end(), w.begin(), w.end());
     rounds = std::max(rounds, v.size());
   }
   std::vector<GemmTask> tasks;

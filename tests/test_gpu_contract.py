@@ -94,7 +94,7 @@ def test_perblock_path_matches_grouped(gpu, c2_pair):
     assert d <= 1e-12 * C1.data.abs().max().item()
 
 
-TILINGS = [(sh, kc, xc) for sh in (0, 1, 2, 3) for kc in (0, 1, 3, -1) for xc in (0, 1)]
+TILINGS = [(sh, kc, xc) for sh in (0, 1, 2, 3) for kc in (0, 1, 3, -1, -2) for xc in (0, 1)]
 
 
 def _last_pair_plan():
@@ -127,6 +127,33 @@ def test_contract_tilings_agree(gpu, c2_pair, mode, tol):
         plan.set_tiling(0, 0, 0, 0)
     with pytest.raises(Exception, match="tile shape"):
         plan.set_tiling(0, 4, 0, 0)
+
+
+def test_balanced_split_schedule_matches_default(gpu):
+    """The balanced two-wave split (kchunk = -2) on blocks whose K is much
+    longer than the mean work per workgroup slot: a head of each long tile
+    in the first wave, tail pieces after it, partials summed in K order;
+    every tile shape and both task orders against the whole-K result."""
+    from tenax_amd import contract
+
+    U1 = oc.U1()
+    rng = np.random.default_rng(8)
+    k = np.repeat(np.array([0, 1], np.int32), [1500, 700])  # long K blocks
+    A = (oc.Leg(U1, np.array([0] * 70 + [1] * 30, np.int32), oc.IN, "a"), oc.Leg(U1, k, oc.OUT, "k"))
+    B = (oc.Leg(U1, k, oc.IN, "k"), oc.Leg(U1, np.array([0] * 90 + [1] * 20, np.int32), oc.OUT, "b"))
+    oA, oB = oc.SymT.random_normal(A, (8, 1)), oc.SymT.random_normal(B, (8, 2))
+    Ad, Bd = from_oracle(oA), from_oracle(oB)
+    C0 = contract(Ad, Bd)
+    plan = _last_pair_plan()
+    ref = np.einsum("ak,kb->ab", oA.todense(), oB.todense())
+    for sh in (0, 1, 2, 3):
+        for xc in (0, 1):
+            plan.set_tiling(0, sh, -2, xc)
+            C = contract(Ad, Bd)
+            np.testing.assert_allclose(C.todense().cpu().numpy(), ref, rtol=0, atol=1e-11 * np.abs(ref).max())
+            d = (C.data - C0.data).abs().max().item()
+            assert d <= 1e-13 * C0.data.abs().max().item(), (sh, xc)
+    plan.set_tiling(0, 0, 0, 0)
 
 
 def test_contract_needs_transposes_and_swap(gpu):

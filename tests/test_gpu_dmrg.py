@@ -105,7 +105,7 @@ def test_matvec_tilings_agree(gpu, state8):
     y0 = prob.matvec(prob.x, 0).clone()
     for st in (1, 3):
         for sh in (0, 1, 2, 3):
-            for kc in (0, 1, -1):
+            for kc in (0, 1, -1, -2):
                 for xc in (0, 1):
                     prob.plan.set_tiling(st, sh, kc, xc)
                     y = prob.matvec(prob.x, 0)

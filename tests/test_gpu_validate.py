@@ -16,7 +16,7 @@ from tests.helpers import from_oracle
 
 pytestmark = pytest.mark.gpu
 
-TILINGS = [(sh, kc, xc) for sh in (0, 1, 2, 3) for kc in (0, 1, -1) for xc in (0, 1)]
+TILINGS = [(sh, kc, xc) for sh in (0, 1, 2, 3) for kc in (0, 1, -1, -2) for xc in (0, 1)]
 
 
 @pytest.fixture(autouse=True)
