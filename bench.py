@@ -99,6 +99,25 @@ def time_events(fn, reps):
     return s.elapsed_time(e) / reps  # ms
 
 
+def capture_matvecs(prob, n):
+    """One torch.cuda graph holding n consecutive fused matvecs of prob.x
+    (recorded on a side stream); returns (graph, output of the last)."""
+    import torch
+
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        prob.matvec(prob.x, 0)
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g, stream=s):
+            for _ in range(n):
+                y = prob.matvec(prob.x, 0)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    return g, y
+
+
 def stage_times(prob, reps, compute=0):
     """Average duration of each fused-matvec stage launched alone (HIP events, current stream)."""
     import ctypes as C
@@ -625,12 +644,23 @@ def main():
     flops = stats["flops"]
     log(f"matvec plan: flops={flops:.3e} pairs={stats['n_pairs']} tasks={stats['n_tasks']}")
 
-    # fused FP64 matvec: the timed steps
+    # fused FP64 matvec: the timed steps are replayed from one captured graph
+    # of `steps` matvecs, the way the sweep's Lanczos solves run them (each
+    # solve is one captured graph); the same matvecs launched eagerly are
+    # timed as well
     for _ in range(args.warmup):
         prob.matvec(prob.x, 0)
-    ms = time_events(lambda: prob.matvec(prob.x, 0), args.steps)
+    ms_eager = time_events(lambda: prob.matvec(prob.x, 0), args.steps)
+    mv_graph, y_graph = capture_matvecs(prob, args.steps)
+    mv_graph.replay()
+    ms = time_events(mv_graph.replay, 1) / args.steps
+    y_eager = prob.matvec(prob.x, 0)
+    torch.cuda.synchronize()
+    graph_equal = bool(torch.equal(y_graph, y_eager))
+    del mv_graph, y_graph
     gflops = flops / (ms * 1e-3) / 1e9
-    log(f"fused f64 matvec {ms:.3f} ms  {gflops:.1f} GFLOP/s")
+    log(f"fused f64 matvec {ms:.3f} ms  {gflops:.1f} GFLOP/s (captured; eager launches {ms_eager:.3f} ms, "
+        f"outputs equal {graph_equal})")
 
     # per-stage kernel times -> roofline of the dominant kernel
     st_ms, st_fl = stage_times(prob, max(5, args.steps // 2))
@@ -728,7 +758,13 @@ def main():
                      "traffic_source": "profiles/dmrg_matvec_chi2048_traffic.md (rocprofv3 --pmc FETCH_SIZE x2.0 + "
                                        "WRITE_SIZE x1.0, copy-calibrated)" if traffic else None,
                      "stage_ms": st_ms, "stage_flops": st_fl},
-        "per_block_gpu": {"ms": ms_pb, "gflops": flops / (ms_pb * 1e-3) / 1e9, "speedup_fused": ms_pb / ms},
+        "matvec_timing": {"captured_ms": ms, "eager_ms": ms_eager, "eager_gflops": flops / (ms_eager * 1e-3) / 1e9,
+                          "captured_equals_eager": graph_equal,
+                          "method": f"{args.steps} matvecs replayed from one captured graph (HIP events around the "
+                                    f"replay, synchronised before and after); eager: the same matvecs launched "
+                                    f"one by one"},
+        "per_block_gpu": {"ms": ms_pb, "gflops": flops / (ms_pb * 1e-3) / 1e9, "speedup_fused": ms_pb / ms,
+                          "speedup_fused_eager": ms_pb / ms_eager},
         "reduced_precision": extra,
         "lanczos_50_steps_s": lz_s,
         "lanczos_50_steps_graph": {"replay_s": lzg_s, "capture_instantiate_s": cap_s,

@@ -2127,6 +2127,9 @@ int tnx_lanczos_graph_rebind(tnx_graph *graph, const tnx_plan *matvec_plan, cons
     tnx_graph_s &G = *graph;
     const FusedPlan &P = static_cast<const FusedPlan &>(*matvec_plan);
     TNX_CHECK_HIP(hipStreamSynchronize((hipStream_t)stream));  // no replay of the old graph in flight
+    // until the new recording is in place the graph must not replay: its
+    // executable still holds the previous plan's work lists
+    G.gen = UINT64_MAX;
     if (&P != G.plan) {
       P.live_graphs.fetch_add(1);
       G.plan->live_graphs.fetch_sub(1);
@@ -2158,7 +2161,8 @@ int tnx_graph_launch(tnx_graph *graph, void *stream) {
   return guarded([&] {
     TNX_REQUIRE(graph && graph->exec, TNX_ERR_ARG, "null graph");
     TNX_REQUIRE(graph->plan->generation == graph->gen, TNX_ERR_ARG,
-                "the plan's tiling changed after capture; call tnx_lanczos_graph_update first");
+                "the plan's tiling changed after capture (or a rebind failed); call tnx_lanczos_graph_update "
+                "or tnx_lanczos_graph_rebind first");
     TNX_CHECK_HIP(hipGraphLaunch(graph->exec, (hipStream_t)stream));
   });
 }
