@@ -38,6 +38,7 @@ METRIC = "DMRG matvec GFLOP/s & sweep s, fused vs per-block GPU at |ΔE|<1e-10"
 PEAK_FP64_MFMA_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 GEMM_ONLY = 16  # TNX_STAGE_GEMM_ONLY (include/tnx_dmrg.h)
+REDUCE_ONLY = 32  # TNX_STAGE_REDUCE_ONLY
 
 
 def log(*a):
@@ -86,11 +87,15 @@ def stage_kernel(prob, stage):
     return f"grouped_gemm_kernel<{lay},0,{bm},{bn},false>", t["n_tasks"], t
 
 
-def time_events(fn, reps):
+def time_events(fn, reps, warm=0):
+    """Average duration (ms) of fn over reps calls, HIP events on the current
+    stream, after warm untimed calls launched back to back with them."""
     import torch
 
     s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
+    for _ in range(warm):
+        fn()
     s.record()
     for _ in range(reps):
         fn()
@@ -119,7 +124,13 @@ def capture_matvecs(prob, n):
 
 
 def stage_times(prob, reps, compute=0):
-    """Average duration of each fused-matvec stage launched alone (HIP events, current stream)."""
+    """Average duration (ms) of each kernel of the fused matvec (coefficients,
+    stage-1 GEMM, its split-K reduction, mix, stage-3 GEMM, its reduction),
+    each launched reps times back to back after 3 untimed launches, HIP events
+    on the current stream around the reps; their sum is compared with the
+    matvec time by the caller.  (Events recorded between the kernels of one
+    matvec would add a pipeline drain and cache write-back per event, which
+    inflates every kernel by 10-30 us.)"""
     import ctypes as C
 
     import torch
@@ -131,17 +142,17 @@ def stage_times(prob, reps, compute=0):
     y = torch.empty_like(prob.x)
     args = lambda st: (prob.plan.ptr, st, _lib.dptr(L.data), _lib.dptr(Wl.data), _lib.dptr(Wr.data),
                        _lib.dptr(R.data), _lib.dptr(prob.x), _lib.dptr(y), compute, _lib.stream_handle())
-    for st in range(4):  # populate intermediates once
-        _lib.check(lib.tnx_matvec_execute_stage(*args(st)))
-    out = {}
-    for st in range(4):
-        out[st] = time_events(lambda: _lib.check(lib.tnx_matvec_execute_stage(*args(st))), reps)
-    # the GEMM kernel of stages 1 and 3 alone (without a split-K reduction):
-    # the launch that rocprofv3 lists as the dominant kernel
-    for st in (1, 3):
-        out[f"{st}_gemm"] = time_events(lambda: _lib.check(lib.tnx_matvec_execute_stage(*args(st | GEMM_ONLY))), reps)
-    for st in range(4):  # leave complete intermediates behind
-        _lib.check(lib.tnx_matvec_execute_stage(*args(st)))
+    seq = [("0", 0), ("1_gemm", 1 | GEMM_ONLY), ("1_reduce", 1 | REDUCE_ONLY), ("2", 2), ("3_gemm", 3 | GEMM_ONLY),
+           ("3_reduce", 3 | REDUCE_ONLY)]
+    run = lambda st: _lib.check(lib.tnx_matvec_execute_stage(*args(st)))
+    for _, st in seq:  # populate the intermediates
+        run(st)
+    out = {name: time_events(lambda: run(st), reps, warm=3) for name, st in seq}
+    out["1"] = out["1_gemm"] + out["1_reduce"]
+    out["3"] = out["3_gemm"] + out["3_reduce"]
+    out["sum"] = sum(out[name] for name, _ in seq)
+    for _, st in seq:  # leave complete intermediates behind
+        run(st)
     f4 = (C.c_double * 4)()
     _lib.check(lib.tnx_fused_stage_flops(prob.plan.ptr, f4))
     return out, list(f4)
@@ -663,7 +674,7 @@ def main():
         f"outputs equal {graph_equal})")
 
     # per-stage kernel times -> roofline of the dominant kernel
-    st_ms, st_fl = stage_times(prob, max(5, args.steps // 2))
+    st_ms, st_fl = stage_times(prob, max(10, args.steps))
     dom = max((1, 3), key=lambda s: st_ms[f"{s}_gemm"])
     ach = st_fl[dom] / (st_ms[f"{dom}_gemm"] * 1e-3) / 1e12
     log(f"stages ms {st_ms}  flops {st_fl}")
@@ -755,6 +766,9 @@ def main():
                      "achieved": ach, "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
                      "frac": ach / PEAK_FP64_MFMA_TFLOPS, "traffic": traffic,
                      "kernel_avg_us": st_ms[f"{dom}_gemm"] * 1e3,
+                     "kernel_timing": f"each matvec kernel launched {max(10, args.steps)} times back to back after "
+                                      f"3 untimed launches, HIP events around them; stage_ms['sum'] (all kernels) "
+                                      f"against ms_per_step checks that they run as fast inside the matvec",
                      "traffic_source": "profiles/dmrg_matvec_chi2048_traffic.md (rocprofv3 --pmc FETCH_SIZE x2.0 + "
                                        "WRITE_SIZE x1.0, copy-calibrated)" if traffic else None,
                      "stage_ms": st_ms, "stage_flops": st_fl},

@@ -45,8 +45,12 @@ int tnx_matvec_execute(const tnx_plan *plan, const double *L, const double *Wl, 
  * 3 = .R grouped GEMM.  Stages read the intermediates the previous stage left
  * in the plan, so stage k is meaningful after stages 0..k-1 ran once.
  * stage | TNX_STAGE_GEMM_ONLY runs the GEMM kernel of stage 1 or 3 without
- * its split-K reduction (its output is then incomplete: timing only). */
+ * its split-K reduction (its output is then incomplete), and
+ * stage | TNX_STAGE_REDUCE_ONLY that reduction alone (nothing when the stage's
+ * tiling has no split tiles): the two in sequence equal the whole stage, so
+ * each kernel can be timed in place inside a matvec. */
 #define TNX_STAGE_GEMM_ONLY 16
+#define TNX_STAGE_REDUCE_ONLY 32
 int tnx_matvec_execute_stage(const tnx_plan *plan, int32_t stage, const double *L,
                              const double *Wl, const double *Wr, const double *R,
                              const double *x, double *y, int32_t compute, void *stream);

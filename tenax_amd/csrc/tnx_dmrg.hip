@@ -1906,7 +1906,16 @@ int tnx_matvec_execute_stage(const tnx_plan *plan, int32_t stage, const double *
     hipStream_t s = (hipStream_t)stream;
     if (P.n_terms == 0) return;
     const bool gemm_only = (stage & TNX_STAGE_GEMM_ONLY) != 0;
-    switch (stage & ~TNX_STAGE_GEMM_ONLY) {
+    const bool reduce_only = (stage & TNX_STAGE_REDUCE_ONLY) != 0;
+    TNX_REQUIRE(!(gemm_only && reduce_only), TNX_ERR_ARG, "TNX_STAGE_GEMM_ONLY and TNX_STAGE_REDUCE_ONLY exclude each other");
+    const int32_t st = stage & ~(TNX_STAGE_GEMM_ONLY | TNX_STAGE_REDUCE_ONLY);
+    if (reduce_only) {
+      TNX_REQUIRE(st == 1 || st == 3, TNX_ERR_ARG, "TNX_STAGE_REDUCE_ONLY applies to stage 1 or 3");
+      launch_split_reduce(st == 1 ? P.s1 : P.s2, st == 1 ? P.X.p : y, false, s);
+      TNX_CHECK_HIP(hipGetLastError());
+      return;
+    }
+    switch (st) {
       case 0:
         launch_coef(P.n_terms, P.cspec.p, Wl, Wr, P.coef.p, s);
         break;
@@ -1921,7 +1930,7 @@ int tnx_matvec_execute_stage(const tnx_plan *plan, int32_t stage, const double *
         launch_grouped_gemm(P.s2, P.Y.p, R, y, compute, false, s, !gemm_only);
         break;
       default:
-        throw Error{TNX_ERR_ARG, "stage must be 0..3 (optionally | TNX_STAGE_GEMM_ONLY)"};
+        throw Error{TNX_ERR_ARG, "stage must be 0..3 (optionally | TNX_STAGE_GEMM_ONLY or TNX_STAGE_REDUCE_ONLY)"};
     }
     TNX_CHECK_HIP(hipGetLastError());
   });

@@ -581,7 +581,12 @@ void launch_grouped_gemm(const GemmProgram &prog, const double *A, const double 
   }
   launch_any(prog.a_lay, prog.b_lay, prog.signs, prog.cfg.shape, compute, prog.n_vtasks, s, prog.vtasks.p, prog.outs.p,
              prog.pairs.p, A, B, C, prog.ws.p, acc, stop);
-  if (reduce && prog.n_red > 0) {
+  if (reduce) launch_split_reduce(prog, C, accumulate, s, stop);
+}
+
+void launch_split_reduce(const GemmProgram &prog, double *C, bool accumulate, hipStream_t s, const int32_t *stop) {
+  const int acc = accumulate ? 1 : 0;
+  if (prog.tiled && prog.n_red > 0) {
     const dim3 g(prog.n_red), b(256);
     switch (prog.cfg.shape) {
       case 0: hipLaunchKernelGGL((split_reduce_kernel<64, 64>), g, b, 0, s, prog.red.p, prog.outs.p, prog.ws.p, C, acc, stop); break;

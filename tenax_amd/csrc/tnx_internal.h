@@ -194,6 +194,9 @@ void set_tiling(GemmProgram &prog, const TileCfg &cfg, hipStream_t s);
 void launch_grouped_gemm(const GemmProgram &prog, const double *A, const double *B, double *C,
                          int compute, bool accumulate, hipStream_t s, bool reduce = true,
                          const int32_t *stop = nullptr);
+// The split-K reduction of a tiled program alone (nothing when it has no split tiles).
+void launch_split_reduce(const GemmProgram &prog, double *C, bool accumulate, hipStream_t s,
+                         const int32_t *stop = nullptr);
 // Launch a prepared program slice (tasks/outs/pairs already on the device).
 void launch_prepared(int a_lay, int b_lay, bool signs, int compute, int n_tasks, hipStream_t s,
                      const GemmTask *tasks, const GemmOut *outs, const GemmPair *pairs, const double *A,

@@ -116,6 +116,42 @@ def test_matvec_tilings_agree(gpu, state8):
         prob.plan.set_tiling(2, 0, 0, 0)
 
 
+@pytest.mark.parametrize("kc", [0, 1, -2])
+def test_matvec_stage_by_stage_equals_matvec(gpu, state8, kc):
+    """The matvec launched kernel by kernel through tnx_matvec_execute_stage
+    (each GEMM with TNX_STAGE_GEMM_ONLY, then its reduction with
+    TNX_STAGE_REDUCE_ONLY) gives the fused matvec's output bit for bit, with
+    and without split K."""
+    from tenax_amd import _lib
+
+    mpo, mps = state8
+    prob, _ = _problem(mpo, mps, 3)
+    lib = _lib.load()
+    for st in (1, 3):
+        prob.plan.set_tiling(st, 0, kc, 1)
+    try:
+        y0 = prob.matvec(prob.x, 0).clone()
+        L, Wl, Wr, R = prob.envs
+        y = torch.full_like(prob.x, float("nan"))
+        for st in (0, 1 | 16, 1 | 32, 2, 3 | 16, 3 | 32):
+            _lib.check(lib.tnx_matvec_execute_stage(prob.plan.ptr, st, _lib.dptr(L.data), _lib.dptr(Wl.data),
+                                                    _lib.dptr(Wr.data), _lib.dptr(R.data), _lib.dptr(prob.x),
+                                                    _lib.dptr(y), 0, _lib.stream_handle()))
+        torch.cuda.synchronize()
+        assert torch.equal(y, y0)
+        with pytest.raises(Exception, match="exclude each other"):
+            _lib.check(lib.tnx_matvec_execute_stage(prob.plan.ptr, 3 | 48, _lib.dptr(L.data), _lib.dptr(Wl.data),
+                                                    _lib.dptr(Wr.data), _lib.dptr(R.data), _lib.dptr(prob.x),
+                                                    _lib.dptr(y), 0, _lib.stream_handle()))
+        with pytest.raises(Exception, match="stage 1 or 3"):
+            _lib.check(lib.tnx_matvec_execute_stage(prob.plan.ptr, 2 | 32, _lib.dptr(L.data), _lib.dptr(Wl.data),
+                                                    _lib.dptr(Wr.data), _lib.dptr(R.data), _lib.dptr(prob.x),
+                                                    _lib.dptr(y), 0, _lib.stream_handle()))
+    finally:
+        for st in (1, 3):
+            prob.plan.set_tiling(st, 0, 0, 0)
+
+
 def test_autotune_picks_a_candidate(gpu, monkeypatch):
     """The autotuner times the candidates on the plan's buffers and applies the fastest."""
     from tenax_amd.contraction import autotune
