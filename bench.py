@@ -124,13 +124,15 @@ def capture_matvecs(prob, n):
 
 
 def stage_times(prob, reps, compute=0):
-    """Average duration (ms) of each kernel of the fused matvec (coefficients,
-    stage-1 GEMM, its split-K reduction, mix, stage-3 GEMM, its reduction),
-    each launched reps times back to back after 3 untimed launches, HIP events
-    on the current stream around the reps; their sum is compared with the
-    matvec time by the caller.  (Events recorded between the kernels of one
-    matvec would add a pipeline drain and cache write-back per event, which
-    inflates every kernel by 10-30 us.)"""
+    """Average duration (ms) of each kernel of the fused matvec: coefficients
+    ("0"), stage-1 GEMM and its split-K reduction ("1_gemm", "1_reduce"), mix
+    ("2"), stage-3 GEMM and reduction ("3_gemm", "3_reduce"); each launched
+    reps times back to back after 3 untimed launches, HIP events on the
+    current stream around the reps.  "sum" adds them, to be compared with the
+    matvec time.  (Events recorded between the kernels of one matvec would add
+    a pipeline drain per event, 10-30 us per kernel; timing by leaving one
+    kernel out of a captured graph of matvecs also removes one inter-kernel
+    gap, ~7 us, per kernel.)"""
     import ctypes as C
 
     import torch
@@ -144,15 +146,22 @@ def stage_times(prob, reps, compute=0):
                        _lib.dptr(R.data), _lib.dptr(prob.x), _lib.dptr(y), compute, _lib.stream_handle())
     seq = [("0", 0), ("1_gemm", 1 | GEMM_ONLY), ("1_reduce", 1 | REDUCE_ONLY), ("2", 2), ("3_gemm", 3 | GEMM_ONLY),
            ("3_reduce", 3 | REDUCE_ONLY)]
+    split = {1: prob.plan.tiling(1)["kchunk"] != 0, 3: prob.plan.tiling(3)["kchunk"] != 0}
     run = lambda st: _lib.check(lib.tnx_matvec_execute_stage(*args(st)))
     for _, st in seq:  # populate the intermediates
         run(st)
-    out = {name: time_events(lambda: run(st), reps, warm=3) for name, st in seq}
+    out = {}
+    for name, st in seq:
+        if name.endswith("_reduce") and not split[int(name[0])]:
+            out[name] = 0.0  # whole-K tiling: no reduction launch
+        else:
+            out[name] = time_events(lambda: run(st), reps, warm=3)
     out["1"] = out["1_gemm"] + out["1_reduce"]
     out["3"] = out["3_gemm"] + out["3_reduce"]
     out["sum"] = sum(out[name] for name, _ in seq)
     for _, st in seq:  # leave complete intermediates behind
         run(st)
+    torch.cuda.synchronize()
     f4 = (C.c_double * 4)()
     _lib.check(lib.tnx_fused_stage_flops(prob.plan.ptr, f4))
     return out, list(f4)
@@ -768,7 +777,7 @@ def main():
                      "kernel_avg_us": st_ms[f"{dom}_gemm"] * 1e3,
                      "kernel_timing": f"each matvec kernel launched {max(10, args.steps)} times back to back after "
                                       f"3 untimed launches, HIP events around them; stage_ms['sum'] (all kernels) "
-                                      f"against ms_per_step checks that they run as fast inside the matvec",
+                                      f"against ms_per_step",
                      "traffic_source": "profiles/dmrg_matvec_chi2048_traffic.md (rocprofv3 --pmc FETCH_SIZE x2.0 + "
                                        "WRITE_SIZE x1.0, copy-calibrated)" if traffic else None,
                      "stage_ms": st_ms, "stage_flops": st_fl},
