@@ -123,16 +123,59 @@ def capture_matvecs(prob, n):
     return g, y
 
 
-def stage_times(prob, reps, compute=0):
-    """Average duration (ms) of each kernel of the fused matvec: coefficients
-    ("0"), stage-1 GEMM and its split-K reduction ("1_gemm", "1_reduce"), mix
-    ("2"), stage-3 GEMM and reduction ("3_gemm", "3_reduce"); each launched
-    reps times back to back after 3 untimed launches, HIP events on the
-    current stream around the reps.  "sum" adds them, to be compared with the
-    matvec time.  (Events recorded between the kernels of one matvec would add
-    a pipeline drain per event, 10-30 us per kernel; timing by leaving one
-    kernel out of a captured graph of matvecs also removes one inter-kernel
-    gap, ~7 us, per kernel.)"""
+class FenceFreeEvents:
+    """HIP timing events created with hipEventDisableSystemFence: recording one
+    between two kernels does not write back and invalidate the caches (a
+    default event's system-scope release does, which slows the next kernel by
+    10-30 us), so events between the kernels of a matvec time each kernel in
+    place."""
+
+    FLAG = 0x20000000  # hipEventDisableSystemFence (hip/hip_runtime_api.h)
+
+    def __init__(self, n):
+        import ctypes as C
+
+        self.C = C
+        self.hip = C.CDLL("libamdhip64.so.7")  # the runtime torch and libtnx already share
+        self.ev = []
+        for _ in range(n):
+            e = C.c_void_p()
+            if self.hip.hipEventCreateWithFlags(C.byref(e), C.c_uint(self.FLAG)) != 0:
+                raise RuntimeError("hipEventCreateWithFlags failed")
+            self.ev.append(e)
+
+    def record(self, i, stream):
+        if self.hip.hipEventRecord(self.ev[i], stream) != 0:
+            raise RuntimeError("hipEventRecord failed")
+
+    def ms(self, i, j):
+        t = self.C.c_float()
+        if self.hip.hipEventElapsedTime(self.C.byref(t), self.ev[i], self.ev[j]) != 0:
+            raise RuntimeError("hipEventElapsedTime failed")
+        return t.value
+
+    def close(self):
+        for e in self.ev:
+            self.hip.hipEventDestroy(e)
+        self.ev = []
+
+
+def stage_times(prob, reps, matvec_ms, compute=0):
+    """Average duration (ms) of each kernel of the fused matvec inside the
+    matvec: coefficients ("0"), stage-1 GEMM and its split-K reduction
+    ("1_gemm", "1_reduce"), mix ("2"), stage-3 GEMM and reduction ("3_gemm",
+    "3_reduce").
+
+    reps matvecs are launched kernel by kernel on the current stream (after 3
+    untimed ones) with a fence-free HIP event before and after every kernel
+    ("events": per-kernel averages, "matvec" the first to the last event).
+    The events themselves lengthen the matvec by -2% to +9% from machine to
+    machine, so each kernel is given its event share of the matvec's own
+    time: kernel = events[kernel] * matvec_ms / events["matvec"], where
+    matvec_ms is the matvec launched eagerly without events ("sum" adds the
+    kernels and equals matvec_ms).  "isolated": each kernel launched reps
+    times back to back on its own (default events around the reps), which has
+    run the GEMMs from 10% faster to 20% slower than inside the matvec."""
     import ctypes as C
 
     import torch
@@ -142,23 +185,40 @@ def stage_times(prob, reps, compute=0):
     lib = _lib.load()
     L, Wl, Wr, R = prob.envs
     y = torch.empty_like(prob.x)
+    stream = _lib.stream_handle()
     args = lambda st: (prob.plan.ptr, st, _lib.dptr(L.data), _lib.dptr(Wl.data), _lib.dptr(Wr.data),
-                       _lib.dptr(R.data), _lib.dptr(prob.x), _lib.dptr(y), compute, _lib.stream_handle())
+                       _lib.dptr(R.data), _lib.dptr(prob.x), _lib.dptr(y), compute, stream)
     seq = [("0", 0), ("1_gemm", 1 | GEMM_ONLY), ("1_reduce", 1 | REDUCE_ONLY), ("2", 2), ("3_gemm", 3 | GEMM_ONLY),
            ("3_reduce", 3 | REDUCE_ONLY)]
     split = {1: prob.plan.tiling(1)["kchunk"] != 0, 3: prob.plan.tiling(3)["kchunk"] != 0}
+    timed = [(name, st) for name, st in seq if not (name.endswith("_reduce") and not split[int(name[0])])]
     run = lambda st: _lib.check(lib.tnx_matvec_execute_stage(*args(st)))
-    for _, st in seq:  # populate the intermediates
-        run(st)
-    out = {}
-    for name, st in seq:
-        if name.endswith("_reduce") and not split[int(name[0])]:
-            out[name] = 0.0  # whole-K tiling: no reduction launch
-        else:
-            out[name] = time_events(lambda: run(st), reps, warm=3)
+    for _ in range(3):  # populate the intermediates, warm up
+        for _, st in seq:
+            run(st)
+    torch.cuda.synchronize()
+    n = len(timed) + 1
+    ev = FenceFreeEvents(reps * n)
+    try:
+        for r in range(reps):
+            ev.record(r * n, stream)
+            for i, (_, st) in enumerate(timed):
+                run(st)
+                ev.record(r * n + i + 1, stream)
+        torch.cuda.synchronize()
+        events = {name: sum(ev.ms(r * n + i, r * n + i + 1) for r in range(reps)) / reps
+                  for i, (name, _) in enumerate(timed)}
+        events["matvec"] = sum(ev.ms(r * n, r * n + n - 1) for r in range(reps)) / reps
+    finally:
+        ev.close()
+    scale = matvec_ms / events["matvec"]
+    out = {name: events.get(name, 0.0) * scale for name, _ in seq}  # no reduction launch for a whole-K tiling
+    out["events"] = events
+    out["matvec_eager"] = matvec_ms
     out["1"] = out["1_gemm"] + out["1_reduce"]
     out["3"] = out["3_gemm"] + out["3_reduce"]
     out["sum"] = sum(out[name] for name, _ in seq)
+    out["isolated"] = {name: time_events(lambda: run(st), reps, warm=3) for name, st in timed}
     for _, st in seq:  # leave complete intermediates behind
         run(st)
     torch.cuda.synchronize()
@@ -353,14 +413,18 @@ def segment_comparison(pd, mps, mpo, chi, reorth, n_bonds=20):
     return out
 
 
-def run_c3(pd, L=100, chi=2048, sweeps=6, seed=2, reorth=True):
+def run_c3(pd, L=100, chi=2048, sweeps=6, seed=2, reorth=True, compute="f64"):
     """The L=100, chi=2048 U(1) Heisenberg DMRG from build_random_symmetric_mps(L, 4, seed)
-    (dmrg.py:125-305, 1415-1523): per-sweep energies, truncation errors, seconds and phases."""
+    (dmrg.py:125-305, 1415-1523): per-sweep energies, truncation errors, seconds and phases.
+    compute: the arithmetic of the block GEMMs (matvecs and environments): "f64"
+    FP64 MFMA, "f32" FP32 MFMA, "split_f32" compensated FP32; Lanczos
+    recurrences, re-orthogonalisation, tridiagonal solve and SVD stay FP64."""
     import torch
 
     from tenax_amd import DMRGConfig, build_mpo_heisenberg, build_random_symmetric_mps, dmrg
 
-    cfg = DMRGConfig(max_bond_dim=chi, num_sweeps=sweeps, lanczos_max_iter=50, lanczos_reorth=reorth)
+    cfg = DMRGConfig(max_bond_dim=chi, num_sweeps=sweeps, lanczos_max_iter=50, lanczos_reorth=reorth,
+                     compute=compute)
     mpo_tn = build_mpo_heisenberg(L)
     mps_tn = build_random_symmetric_mps(L, 4, seed=seed)
     pd.TIMERS.clear()
@@ -377,9 +441,10 @@ def run_c3(pd, L=100, chi=2048, sweeps=6, seed=2, reorth=True):
     mps = [res.mps.get_tensor(i) for i in range(L)]
     mpo = [mpo_tn.get_tensor(i) for i in range(L)]
     bond_dims = [int(len(mps[i].indices[-1].charges)) for i in range(L - 1)]
+    arith = {"f64": "FP64 MFMA", "f32": "FP32 MFMA", "split_f32": "split-FP32 MFMA"}[compute]
     out = {"workload": f"U(1) Heisenberg chain L={L}, Jz=Jxy=1, MPO bond 5, build_random_symmetric_mps(L, 4, "
                        f"seed={seed}), max_bond_dim={chi}, {sweeps} sweeps, Lanczos 50 steps"
-                       f"{' with FP64 re-orthogonalisation' if reorth else ''}, FP64 MFMA matvec, FP64 SVD",
+                       f"{' with FP64 re-orthogonalisation' if reorth else ''}, {arith} block GEMMs, FP64 SVD",
            "seconds": total, "sweep_seconds": list(pd.LAST_RUN["sweep_seconds"]),
            "energies_per_sweep": list(res.energies_per_sweep), "energy": res.energy,
            "energy_per_site": res.energy / L, "converged": res.converged,
@@ -683,7 +748,7 @@ def main():
         f"outputs equal {graph_equal})")
 
     # per-stage kernel times -> roofline of the dominant kernel
-    st_ms, st_fl = stage_times(prob, max(10, args.steps))
+    st_ms, st_fl = stage_times(prob, max(10, args.steps), ms_eager)
     dom = max((1, 3), key=lambda s: st_ms[f"{s}_gemm"])
     ach = st_fl[dom] / (st_ms[f"{dom}_gemm"] * 1e-3) / 1e12
     log(f"stages ms {st_ms}  flops {st_fl}")
@@ -775,9 +840,11 @@ def main():
                      "achieved": ach, "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
                      "frac": ach / PEAK_FP64_MFMA_TFLOPS, "traffic": traffic,
                      "kernel_avg_us": st_ms[f"{dom}_gemm"] * 1e3,
-                     "kernel_timing": f"each matvec kernel launched {max(10, args.steps)} times back to back after "
-                                      f"3 untimed launches, HIP events around them; stage_ms['sum'] (all kernels) "
-                                      f"against ms_per_step",
+                     "kernel_timing": f"in place: {max(10, args.steps)} matvecs launched kernel by kernel with a "
+                                      f"fence-free HIP event (hipEventDisableSystemFence) before and after every "
+                                      f"kernel; each kernel's event share of the eager matvec time "
+                                      f"(stage_ms['events'] raw, stage_ms['isolated'] each kernel back to back "
+                                      f"on its own)",
                      "traffic_source": "profiles/dmrg_matvec_chi2048_traffic.md (rocprofv3 --pmc FETCH_SIZE x2.0 + "
                                        "WRITE_SIZE x1.0, copy-calibrated)" if traffic else None,
                      "stage_ms": st_ms, "stage_flops": st_fl},
@@ -813,6 +880,15 @@ def main():
         del cprob
         result["c3"]["centre_bond_reduced_precision"] = c3_reduced_precision(pd, c3_mps, c3_mpo, not args.no_reorth)
         log(f"C3 centre bond reduced precision: {result['c3']['centre_bond_reduced_precision']}")
+        # the whole C3 run again with the block GEMMs in FP32 and in split FP32
+        runs = {}
+        for mode in ("f32", "split_f32"):
+            r, *_ = run_c3(pd, args.L, args.chi, args.c3_sweeps, args.seed, not args.no_reorth, compute=mode)
+            runs[mode] = {"seconds": r["seconds"], "energies_per_sweep": r["energies_per_sweep"],
+                          "energy": r["energy"], "abs_dE_vs_f64_run": abs(r["energy"] - c3["energy"]),
+                          "phases_s": r["phases_s"]}
+            log(f"C3 {mode}: {runs[mode]}")
+        result["c3"]["reduced_precision_runs"] = runs
         cmp = compare_updates(pd, c3_mps, c3_mpo, args.chi, not args.no_reorth)
         result["two_site_updates_centre_s"] = cmp
         log(f"C3 centre updates fused vs per-block: {cmp}")
