@@ -169,11 +169,11 @@ def stage_times(prob, reps, matvec_ms, compute=0):
     reps matvecs are launched kernel by kernel on the current stream (after 3
     untimed ones) with a fence-free HIP event before and after every kernel
     ("events": per-kernel averages, "matvec" the first to the last event).
-    The events themselves lengthen the matvec by -2% to +9% from machine to
-    machine, so each kernel is given its event share of the matvec's own
-    time: kernel = events[kernel] * matvec_ms / events["matvec"], where
-    matvec_ms is the matvec launched eagerly without events ("sum" adds the
-    kernels and equals matvec_ms).  "isolated": each kernel launched reps
+    The events themselves lengthen the matvec by -2% to +12% from machine to
+    machine, so each kernel is given its event share of the timed matvec:
+    kernel = events[kernel] * matvec_ms / events["matvec"], where matvec_ms
+    is the matvec time of the timed steps (replayed from a captured graph,
+    no events; "sum" adds the kernels and equals matvec_ms).  "isolated": each kernel launched reps
     times back to back on its own (default events around the reps), which has
     run the GEMMs from 10% faster to 20% slower than inside the matvec."""
     import ctypes as C
@@ -214,7 +214,7 @@ def stage_times(prob, reps, matvec_ms, compute=0):
     scale = matvec_ms / events["matvec"]
     out = {name: events.get(name, 0.0) * scale for name, _ in seq}  # no reduction launch for a whole-K tiling
     out["events"] = events
-    out["matvec_eager"] = matvec_ms
+    out["matvec"] = matvec_ms
     out["1"] = out["1_gemm"] + out["1_reduce"]
     out["3"] = out["3_gemm"] + out["3_reduce"]
     out["sum"] = sum(out[name] for name, _ in seq)
@@ -748,7 +748,7 @@ def main():
         f"outputs equal {graph_equal})")
 
     # per-stage kernel times -> roofline of the dominant kernel
-    st_ms, st_fl = stage_times(prob, max(10, args.steps), ms_eager)
+    st_ms, st_fl = stage_times(prob, max(10, args.steps), ms)
     dom = max((1, 3), key=lambda s: st_ms[f"{s}_gemm"])
     ach = st_fl[dom] / (st_ms[f"{dom}_gemm"] * 1e-3) / 1e12
     log(f"stages ms {st_ms}  flops {st_fl}")
@@ -842,7 +842,7 @@ def main():
                      "kernel_avg_us": st_ms[f"{dom}_gemm"] * 1e3,
                      "kernel_timing": f"in place: {max(10, args.steps)} matvecs launched kernel by kernel with a "
                                       f"fence-free HIP event (hipEventDisableSystemFence) before and after every "
-                                      f"kernel; each kernel's event share of the eager matvec time "
+                                      f"kernel; each kernel's event share of the timed (captured) matvec time "
                                       f"(stage_ms['events'] raw, stage_ms['isolated'] each kernel back to back "
                                       f"on its own)",
                      "traffic_source": "profiles/dmrg_matvec_chi2048_traffic.md (rocprofv3 --pmc FETCH_SIZE x2.0 + "
