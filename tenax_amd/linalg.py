@@ -305,6 +305,13 @@ def _factor_all(mats, absorb):
 LEADING_TOL = 2e-14
 LEADING_MAX_ITER = 40
 LEADING_MIN_RATIO = 4
+# Matrices of nearly equal shape (e.g. the sectors of one step's instances,
+# whose charge splits differ by a few states) iterate together, zero-padded
+# to the largest shape of their group, when that costs at most this factor
+# in elements: zero rows and columns change neither the singular values nor
+# the singular vectors' restriction to the matrix, and one batched iteration
+# replaces one per shape.
+LEADING_PAD = 1.25
 LEADING_STATS: dict = {"calls": 0, "matrices": 0, "iterations": 0, "fallback": 0, "fb_orth": 0, "fb_svd": 0,
                        "fb_conv": 0, "complete_eigh_fail": 0, "complete_chol_fail": 0}
 # Ritz vectors of singular values below this fraction of the largest are not
@@ -434,8 +441,7 @@ def _leading_isometries(mats, k):
     matrix, s descending; matrices too small for the iteration go through the
     Jordan-Wielandt factorisation and are cut to the same count."""
     out = [None] * len(mats)
-    groups: dict = {}
-    small = []
+    small, iterate = [], []
     big = [j for j, M in enumerate(mats) if min(M.shape) >= LEADING_MIN_RATIO * k]
     norms = {}
     if big:
@@ -445,20 +451,43 @@ def _leading_isometries(mats, k):
         # the iteration works on M / |M|_F; zero or non-finite matrices take
         # the full factorisation
         if j in norms and np.isfinite(norms[j]) and norms[j] > 0:
-            groups.setdefault(tuple(M.shape), []).append(j)
+            iterate.append(j)
         else:
             small.append(j)
+    # groups [R, C, smallest member's elements, members]: largest matrices
+    # first, each joins the first group whose padded shape (the largest rows
+    # and columns of its members) stays within LEADING_PAD times the elements
+    # of every member
+    groups = []
+    for j in sorted(iterate, key=lambda j: (-mats[j].shape[0] * mats[j].shape[1], j)):
+        R, C = mats[j].shape
+        for g in groups:
+            Rg, Cg = max(g[0], R), max(g[1], C)
+            if Rg * Cg <= LEADING_PAD * min(g[2], R * C):
+                g[0], g[1], g[2] = Rg, Cg, min(g[2], R * C)
+                g[3].append(j)
+                break
+        else:
+            groups.append([R, C, R * C, [j]])
     redo = list(small)
     LEADING_STATS["calls"] += 1
-    for (R, C), js in groups.items():
+    for R, C, _, js in groups:
         dev = mats[js[0]].device
         p = 2 * k
-        scale = torch.tensor([norms[j] for j in js], dtype=torch.float64, device=dev)
-        Mb = torch.stack([mats[j] for j in js]) / scale[:, None, None]
+        Mb = torch.zeros((len(js), R, C), dtype=torch.float64, device=dev)
+        rmask = torch.zeros((len(js), R, 1), dtype=torch.float64, device=dev)
+        cmask = torch.zeros((len(js), C, 1), dtype=torch.float64, device=dev)
+        for b, j in enumerate(js):
+            r, c = mats[j].shape
+            Mb[b, :r, :c] = mats[j] / float(norms[j])
+            rmask[b, :r] = 1.0
+            cmask[b, :c] = 1.0
         Mt = Mb.transpose(1, 2).contiguous()
         gen = torch.Generator(device=dev)
         gen.manual_seed(0x51DE)
-        V, _, ok = _orth(torch.randn((len(js), C, p), generator=gen, dtype=torch.float64, device=dev))
+        # random directions (start block, completions) only in the rows and
+        # columns a member owns, so its padding stays exactly zero
+        V, _, ok = _orth(torch.randn((len(js), C, p), generator=gen, dtype=torch.float64, device=dev) * cmask)
         Y = torch.bmm(Mb, V)
         conv = torch.zeros(len(js), dtype=torch.bool, device=dev)
         it = 0
@@ -466,9 +495,9 @@ def _leading_isometries(mats, k):
         svd_failed = False
         while it < LEADING_MAX_ITER and js:
             if complete:
-                U, ok1 = _orth_complete(Y, torch.randn(Y.shape, generator=gen, dtype=Y.dtype, device=dev))
+                U, ok1 = _orth_complete(Y, torch.randn(Y.shape, generator=gen, dtype=Y.dtype, device=dev) * rmask)
                 Z = torch.bmm(Mt, U)
-                V, ok2 = _orth_complete(Z, torch.randn(Z.shape, generator=gen, dtype=Z.dtype, device=dev))
+                V, ok2 = _orth_complete(Z, torch.randn(Z.shape, generator=gen, dtype=Z.dtype, device=dev) * cmask)
                 Rv = torch.bmm(V.transpose(1, 2), Z)  # = V^T M^T U, so R^T = U^T M V
             else:
                 U, _, ok1 = _orth(Y)
@@ -489,7 +518,7 @@ def _leading_isometries(mats, k):
                 js = [j for j, good in zip(js, ok_host) if good]
                 if not js:
                     break
-                Mb, Mt, V = Mb[keep], Mt[keep], V[keep]
+                Mb, Mt, V, rmask, cmask = Mb[keep], Mt[keep], V[keep], rmask[keep], cmask[keep]
                 Y = torch.bmm(Mb, V)
                 ok = torch.ones(len(js), dtype=torch.bool, device=dev)
                 conv = torch.zeros(len(js), dtype=torch.bool, device=dev)
@@ -530,7 +559,8 @@ def _leading_isometries(mats, k):
         sk = s[:, :k] * torch.tensor([norms[j] for j in js], dtype=torch.float64, device=dev)[:, None]
         for b, j in enumerate(js):
             if conv_host[b]:
-                out[j] = (Uk[b], sk[b], Vhk[b])
+                r, c = mats[j].shape
+                out[j] = (Uk[b, :r], sk[b], Vhk[b, :, :c])
             else:
                 redo.append(j)
     if redo:

@@ -131,3 +131,30 @@ def test_truncated_svd_leading_mode_equals_full_mode(gpu):
         b = torch.cat([full[0].block(i).reshape(-1, int(full[0].shapes[i, -1])) for i in rows])
         sv = torch.linalg.svdvals(a.T @ b)
         assert torch.allclose(sv, torch.ones_like(sv), atol=1e-10)
+
+
+def test_nearly_equal_shapes_iterate_in_one_padded_group(monkeypatch):
+    """Sector matrices whose shapes differ by a few rows or columns (instances
+    whose charge splits differ) iterate as one zero-padded batch; each still
+    gets its own leading triplets, unpadded, to LAPACK's accuracy.  A shape
+    too far from the others keeps a group of its own."""
+    rng = np.random.default_rng(17)
+    k = 24
+    shapes = [(400, 400), (396, 404), (390, 388), (402, 398), (250, 260)]
+    Ms = [_matrix(rng, *sh, 10.0) for sh in shapes]
+    groups = []
+    real_orth = L._orth
+
+    def spy(X):
+        groups.append(X.shape)
+        return real_orth(X)
+
+    monkeypatch.setattr(L, "_orth", spy)
+    before = dict(L.LEADING_STATS)
+    out = L._leading_isometries([torch.tensor(M) for M in Ms], k)
+    assert L.LEADING_STATS["fallback"] == before["fallback"]
+    # the first _orth of each group orthonormalises its random start block
+    starts = [g for g in groups if g[2] == 2 * k]
+    assert (4, 404, 2 * k) in starts and (1, 260, 2 * k) in starts
+    for M, (U, s, Vh) in zip(Ms, out):
+        _check(M, U, s, Vh, k)
