@@ -22,6 +22,8 @@ import threading
 
 import torch
 
+from tenax_amd import _lib
+
 # (shape, kchunk, xcd): shape 0 = 64x64, 1 = 128x64, 2 = 64x128; kchunk -1 = split K automatically,
 # -2 = the balanced two-wave split (whole tiles and heads of the long ones at
 # once, short tail pieces after).  Near-ties go to the earliest candidate: the
@@ -73,7 +75,14 @@ def tune(plan, stage: int, run, flops: float, kind: str):
     if cfg is None:
         timed = []
         for c in CANDIDATES:
-            plan.set_tiling(stage, *c)
+            try:
+                plan.set_tiling(stage, *c)
+            except _lib.TnxError as e:
+                # a split whose partial tiles would exceed the workspace cap
+                # is not a candidate for this plan
+                if e.code != _lib.TNX_ERR_CAPACITY:
+                    raise
+                continue
             timed.append((_time(run, REPS), c))
         # near-ties go to the earliest candidate (fixed order), so the choice
         # does not flip between runs on timing noise

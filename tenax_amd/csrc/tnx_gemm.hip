@@ -253,6 +253,7 @@ __global__ __launch_bounds__(GEMM_THREADS) __attribute__((amdgpu_waves_per_eu(
   constexpr int NT = (COMPUTE == 2) ? 2 : 1;  // hi/lo planes for split
   constexpr int AP = BM + 1, BP = BN + 1;
   constexpr int TI = BM / 32, TJ = BN / 32;
+  constexpr bool ONE_STAGE = BM * BN > 8192;
   using T = typename std::conditional<COMPUTE == 0, double, float>::type;
   // Two LDS stages: tile t is read from stage t&1 while tile t+1 is written
   // to the other stage, so each K-tile needs a single barrier.
@@ -439,6 +440,24 @@ __global__ __launch_bounds__(GEMM_THREADS) __attribute__((amdgpu_waves_per_eu(
     }
   };
   auto main_loop = [&](auto guard) {
+    if (ONE_STAGE) {
+      // one register stage (128x128 tiles: a tile's 64 MFMAs per wave cover
+      // the load latency of the next): tile i+1 is in flight from memory
+      // while tile i is computed from LDS
+      Stage<BM, BN> r0;
+      load(r0);
+      stage_store(0, r0);
+      __syncthreads();
+      for (int i = 0; i < n; ++i) {
+        load(r0);
+        compute_tile(i & 1, guard);
+        if (i + 1 < n) {
+          stage_store((i + 1) & 1, r0);
+          __syncthreads();
+        }
+      }
+      return;
+    }
     // two register stages: tile i+1 waits in registers while tile i+2 is in flight
     Stage<BM, BN> r0, r1;
     load(r0);

@@ -581,8 +581,10 @@ void set_tiling(GemmProgram &prog, const TileCfg &cfg, hipStream_t s) {
         slot += nsplit;
       }
   }
-  TNX_REQUIRE(slot * bm * bn <= MAX_WS_DOUBLES, TNX_ERR_CAPACITY,
-              "split-K workspace exceeds its cap; use a larger kchunk");
+  if (slot * bm * bn > MAX_WS_DOUBLES) {
+    prog.cfg = TileCfg{};  // the program keeps the default tiling
+    throw Error{TNX_ERR_CAPACITY, "split-K workspace exceeds its cap; use a larger kchunk"};
+  }
   // each bin longest task first (LPT); ties keep unit order (tiles of one output together)
   auto cost = [](const GemmTask &t, const std::vector<int64_t> &kt) {
     return std::min<int64_t>(t.kt1, std::max<int64_t>(kt[t.out], 1)) - t.kt0;

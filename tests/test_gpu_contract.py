@@ -419,3 +419,39 @@ def test_contract_vjp_matches_dense_chain_rule(gpu, k_first, sym):
     assert np.abs((got_a - ref_a)[ma]).max() < 1e-12 * scale
     assert np.abs((got_b - ref_b)[mb]).max() < 1e-12 * scale
     assert np.all(got_a[~ma] == 0) and np.all(got_b[~mb] == 0)
+
+
+def test_split_over_workspace_cap_is_refused_and_skipped(gpu, monkeypatch):
+    """A split-K tiling whose partial tiles exceed the workspace cap (the
+    balanced split of a 2048^3 one-block contraction in 128x128 tiles) raises
+    TNX_ERR_CAPACITY, leaves the plan on its default tiling (which still
+    contracts correctly), and is skipped as an autotuning candidate."""
+    from tenax_amd import SymmetricTensor, _lib, contract
+    from tenax_amd.contraction import autotune, contractor, plans
+    from tenax_amd.core.index import FlowDirection, TensorIndex
+    from tenax_amd.core.symmetry import U1Symmetry
+
+    U1 = U1Symmetry()
+    n = 2048
+    z = np.zeros(n, np.int32)
+    A = SymmetricTensor.random_normal((TensorIndex(U1, z, FlowDirection.IN, "l"),
+                                       TensorIndex(U1, z, FlowDirection.OUT, "r")), (1, 0))
+    B = SymmetricTensor.random_normal((TensorIndex(U1, z, FlowDirection.IN, "r"),
+                                       TensorIndex(U1, z, FlowDirection.OUT, "s")), (1, 1))
+    C = contract(A, B)
+    ref = (A.data.view(n, n) @ B.data.view(n, n)).reshape(-1)
+    plan = plans.pairwise_plan(A, B, 0, 1, 1, C.indices, 0, None)[0]
+    with pytest.raises(_lib.TnxError, match="workspace exceeds its cap") as err:
+        plan.set_tiling(0, 3, -2, 0)
+    assert err.value.code == _lib.TNX_ERR_CAPACITY
+    assert plan.tiling(0)["shape"] == 0 and plan.tiling(0)["kchunk"] == 0
+    y = torch.empty_like(C.data)
+    contractor._execute_pair(plan, A, B, y)
+    assert float((y - ref).abs().max() / ref.abs().max()) <= 1e-13
+    autotune.clear()
+    monkeypatch.setattr(autotune, "CANDIDATES", [(3, -2, 0), (0, 0, 1)])
+    cfg = autotune.tune(plan, 0, lambda: contractor._execute_pair(plan, A, B, y), plan.stats()["flops"], "test")
+    assert cfg == (0, 0, 1)
+    contractor._execute_pair(plan, A, B, y)
+    assert float((y - ref).abs().max() / ref.abs().max()) <= 1e-13
+    autotune.clear()
