@@ -95,6 +95,21 @@ int tnx_plan_pairwise(const tnx_blocks *a, const tnx_blocks *b, int32_t a_kfirst
                       int32_t has_target, int32_t target, void *stream,
                       tnx_plan **plan);
 
+/* Pairwise plan whose output legs are stored in another order: as
+ * tnx_plan_pairwise (out_legs still describe [free A..., free B...]), and
+ * output leg j of the result is natural leg out_perm[j].  The output blocks
+ * (tnx_plan_output_copy) are keyed, sorted and laid out in that order and the
+ * grouped GEMM (and its split-K reduction) writes every element straight to
+ * its place, where the reference contracts and then transposes
+ * (contractor.py:474-484, tensor.py:832-861).  Bosonic charges only; such a
+ * plan has no per-block path and no adjoint.  TNX_ERR_ARG when a side (the A
+ * or B free legs) needs more than 3 strides after merging the legs that stay
+ * adjacent (callers then transpose the natural result instead). */
+int tnx_plan_pairwise_permuted(const tnx_blocks *a, const tnx_blocks *b, int32_t a_kfirst,
+                               int32_t b_kfirst, int32_t n_contracted, const tnx_leg *out_legs,
+                               int32_t sym_n, int32_t mode, int32_t has_target, int32_t target,
+                               const int32_t *out_perm, void *stream, tnx_plan **plan);
+
 /* Pairwise plan for graded (fermionic) charges: as tnx_plan_pairwise, and
  * each block pair enters with the exchange sign of the reference
  * (contractor.py:206-285 inversion pairs, applied per combination at

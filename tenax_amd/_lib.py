@@ -79,6 +79,8 @@ SIGNATURES = {
                                     _i32, _i32, _i32, _i32, _p, C.POINTER(_p)]),
     "tnx_plan_pairwise_graded": (C.c_int, [C.POINTER(Blocks), C.POINTER(Blocks), _i32, _i32, _i32, C.POINTER(Leg),
                                            _i32, _i32, _i32, _i32, _i32, _PI32, _p, C.POINTER(_p)]),
+    "tnx_plan_pairwise_permuted": (C.c_int, [C.POINTER(Blocks), C.POINTER(Blocks), _i32, _i32, _i32, C.POINTER(Leg),
+                                             _i32, _i32, _i32, _i32, _PI32, _p, C.POINTER(_p)]),
     "tnx_plan_output_info": (C.c_int, [_p, _PI32, _PI32, _PI64]),
     "tnx_plan_output_copy": (C.c_int, [_p, _PI32, _PI32, _PI64]),
     "tnx_plan_stats": (C.c_int, [_p, _PD, _PD, _PI32, _PI32]),

@@ -95,11 +95,28 @@ def _isometries(Ms, roles, max_bond_dim, svd_trunc_err):
     return out
 
 
+def _next_a_layout(P, Q, nxt, nxt_out=None):
+    """Leg order for the result of contract(P, Q) that the contraction with
+    ``nxt`` takes as its A operand without a transpose: the legs it keeps
+    (in the order of that contraction's output ``nxt_out``, or their own),
+    then the legs it contracts with ``nxt``."""
+    lp, lq = P.labels(), Q.labels()
+    labels = [l for l in lp if l not in lq] + [l for l in lq if l not in lp]
+    other = set(nxt.labels())
+    cont = [l for l in labels if l in other]
+    keep = [l for l in labels if l not in other]
+    if nxt_out is not None:
+        keep = [l for l in nxt_out if l in keep]
+    return tuple(keep + cont)
+
+
 def _steps(Ts, roles, max_bond_dim, svd_trunc_err):
     """One coarse-graining step of every instance, in lockstep: each
     contraction runs for all instances at once (contract_many: one grouped
     block-GEMM launch over every instance's block pairs, one launch per leg
-    permutation phase)."""
+    permutation phase).  The chi^5 intermediates are written by the grouped
+    GEMM directly in the leg order the next contraction reads (no transpose
+    of them)."""
     t = _tick("", 0.0)
     Ms = contract_many([(T, T.relabels(roles["env"])) for T in Ts])
     t = _tick("environment", t)
@@ -110,9 +127,12 @@ def _steps(Ts, roles, max_bond_dim, svd_trunc_err):
     Vbs = [Vh.relabel("a", "b").dagger() for _, Vh in isos]
     out = []
     firsts = [T.relabel(*roles["first"]) for T in Ts]
+    seconds = [T.relabels(roles["second"]) for T in Ts]
     for lo, hi in _chunks(list(zip(Uds, firsts))):
-        X = contract_many(list(zip(Uds[lo:hi], firsts[lo:hi])))
-        Y = contract_many([(x, T.relabels(roles["second"])) for x, T in zip(X, Ts[lo:hi])])
+        x_order = _next_a_layout(Uds[lo], firsts[lo], seconds[lo])
+        X = contract_many(list(zip(Uds[lo:hi], firsts[lo:hi])), output_labels=x_order)
+        y_order = _next_a_layout(X[0], seconds[lo], Vbs[lo], roles["out"])
+        Y = contract_many(list(zip(X, seconds[lo:hi])), output_labels=y_order)
         del X
         Tn = contract_many(list(zip(Y, Vbs[lo:hi])), output_labels=roles["out"])
         del Y

@@ -232,6 +232,25 @@ def _is_graded(T) -> bool:
     return bool(T.indices) and T.indices[0].symmetry.is_fermionic
 
 
+# The grouped GEMM writes a requested output leg order directly (a
+# permuted-store plan) when each side of the product -- the A free legs
+# (rows) and the B free legs (columns) -- falls into at most this many runs of
+# legs that stay adjacent in the requested order; otherwise the natural
+# result is transposed.  FUSED_OUTPUT_ORDER = False always transposes.
+FUSED_OUTPUT_ORDER = True
+_STORE_LEGS = 3  # STORE_LEGS of the library
+
+
+def _fused_order_fits(out_axes, nfa: int) -> bool:
+    pos = {leg: j for j, leg in enumerate(out_axes)}
+    n = len(out_axes)
+    for lo, hi in ((0, nfa), (nfa, n)):
+        runs = sum(1 for i in range(lo, hi) if i == lo or pos[i] != pos[i - 1] + 1)
+        if runs > _STORE_LEGS:
+            return False
+    return True
+
+
 class _PairJob:
     """The host-side decisions of one pairwise contraction: which operand
     plays A, the leg permutations that bring A to [free, contracted] (or
@@ -240,7 +259,7 @@ class _PairJob:
     one."""
 
     __slots__ = ("A", "B", "a_axes", "b_axes", "a_kfirst", "nc", "nat", "nat_indices", "out_axes", "mode",
-                 "target", "sign_pairs", "graded")
+                 "target", "sign_pairs", "graded", "nfa")
 
     def __init__(self, A, B, sa, sb, so, out_indices, mode, target):
         if len(set(sa)) != len(sa) or len(set(sb)) != len(sb):
@@ -289,7 +308,13 @@ class _PairJob:
             pos.setdefault(c, len(sa) + i)
         self.sign_pairs = [(pos[x], pos[y]) for x, y in sign_chars]
         self.out_axes = tuple(self.nat.index(c) for c in so) if self.nat != so else None
+        self.nfa = len(fa)
         self.mode, self.target = mode, target
+
+    def fused_order(self) -> bool:
+        """Whether the GEMM should write the requested output order itself."""
+        return (self.out_axes is not None and FUSED_OUTPUT_ORDER and not self.graded and _state["grouped"]
+                and _fused_order_fits(self.out_axes, self.nfa))
 
 
 def contract_pair(A: SymmetricTensor, B: SymmetricTensor, sa: str, sb: str, so: str,
@@ -314,11 +339,20 @@ def contract_pair(A: SymmetricTensor, B: SymmetricTensor, sa: str, sb: str, so: 
     move = (lambda T, axes: _transpose_raw(T, axes, graded=False)) if graded else (lambda T, axes: T.transpose(axes))
     A = job.A if job.a_axes is None else move(job.A, job.a_axes)
     B = job.B if job.b_axes is None else move(job.B, job.b_axes)
+    differentiated = torch.is_grad_enabled() and (A.data.requires_grad or B.data.requires_grad)
+    if job.fused_order() and not differentiated:
+        # the GEMM stores the requested leg order directly
+        plan, keys, shapes, offs, tot = plans.pairwise_plan(A, B, job.a_kfirst, 1, job.nc, job.nat_indices, mode,
+                                                            target, job.sign_pairs, out_perm=job.out_axes)
+        out = torch.empty(tot, dtype=torch.float64, device=A.data.device)
+        if tot:
+            _execute_pair(plan, A, B, out)
+        return SymmetricTensor._wrap(tuple(job.nat_indices[p] for p in job.out_axes), keys, shapes, offs, out)
     plan, keys, shapes, offs, tot = plans.pairwise_plan(A, B, job.a_kfirst, 1, job.nc, job.nat_indices, mode,
                                                         target, job.sign_pairs)
     out = torch.empty(tot, dtype=torch.float64, device=A.data.device)
     C = SymmetricTensor._wrap(job.nat_indices, keys, shapes, offs, out)
-    if torch.is_grad_enabled() and (A.data.requires_grad or B.data.requires_grad):
+    if differentiated:
         Ad = A._with_data(A.data.detach())
         Bd = B._with_data(B.data.detach())
         C = C._with_data(_PairExecFn.apply(A.data, B.data, plan, Ad, Bd, C, None, None, job.nat))
@@ -368,13 +402,16 @@ def contract_pairs_batched(jobs) -> list[SymmetricTensor]:
                     j.a_axes = None
                 else:
                     j.b_axes = None
-    planned = [plans.pairwise_plan(j.A, j.B, j.a_kfirst, 1, j.nc, j.nat_indices, j.mode, j.target, j.sign_pairs)
-               for j in jobs]
+    fused = [j.fused_order() for j in jobs]
+    planned = [plans.pairwise_plan(j.A, j.B, j.a_kfirst, 1, j.nc, j.nat_indices, j.mode, j.target, j.sign_pairs,
+                                   out_perm=j.out_axes if f else None)
+               for j, f in zip(jobs, fused)]
     starts = np.concatenate([[0], np.cumsum([p[4] for p in planned])]).astype(np.int64)
     dev = jobs[0].A.data.device
     buf = torch.empty(int(starts[-1]), dtype=torch.float64, device=dev)
-    res = [SymmetricTensor._wrap(j.nat_indices, p[1], p[2], p[3], buf[starts[k]:starts[k + 1]])
-           for k, (j, p) in enumerate(zip(jobs, planned))]
+    res = [SymmetricTensor._wrap(tuple(j.nat_indices[q] for q in j.out_axes) if f else j.nat_indices,
+                                 p[1], p[2], p[3], buf[starts[k]:starts[k + 1]])
+           for k, (j, p, f) in enumerate(zip(jobs, planned, fused))]
     for kfirst in (0, 1):
         ks = [k for k, j in enumerate(jobs) if j.a_kfirst == kfirst and planned[k][4] > 0]
         if not ks:
@@ -391,7 +428,7 @@ def contract_pairs_batched(jobs) -> list[SymmetricTensor]:
             WORK["flops"] += bplan.stats()["flops"]
             WORK["launches"] += 1
         _lib.check(lib.tnx_batch_execute(bplan.ptr, _lib.COMPUTE[_state["compute"]], _lib.stream_handle()))
-    fin = [k for k, j in enumerate(jobs) if j.out_axes is not None]
+    fin = [k for k, j in enumerate(jobs) if j.out_axes is not None and not fused[k]]
     if fin:
         moved = _batched_transposes([res[k] for k in fin], [jobs[k].out_axes for k in fin])
         for k, T in zip(fin, moved):

@@ -63,15 +63,21 @@ def _legs_sig(indices):
     return tuple((ix.sectors()[0].tobytes(), ix.sectors()[1].tobytes(), int(ix.flow)) for ix in indices)
 
 
-def pairwise_plan(A, B, a_kfirst: int, b_kfirst: int, nc: int, out_indices, mode: int, target, sign_pairs=()):
+def pairwise_plan(A, B, a_kfirst: int, b_kfirst: int, nc: int, out_indices, mode: int, target, sign_pairs=(),
+                  out_perm=None):
     """Plan C = A . B (device planner); returns (Plan, keys, shapes, offsets, total).
 
     ``sign_pairs``: fermionic exchange-sign leg pairs (positions in the
-    concatenated [A legs, B legs]), evaluated per block pair on the device."""
+    concatenated [A legs, B legs]), evaluated per block pair on the device.
+    ``out_perm``: store output leg j as natural leg out_perm[j] (the natural
+    order is [A free legs, B free legs], described by ``out_indices``); the
+    GEMM then writes that order directly (tnx_plan_pairwise_permuted; raises
+    TnxError when the permutation does not fit the store descriptors)."""
     sym = A.indices[0].symmetry if A.indices else out_indices[0].symmetry
     sign_pairs = tuple((int(x), int(y)) for x, y in sign_pairs)
+    out_perm = None if out_perm is None else tuple(int(p) for p in out_perm)
     key = ("pair", _struct_sig(A), _struct_sig(B), A.ndim, B.ndim, a_kfirst, b_kfirst, nc,
-           _legs_sig(out_indices), sym.sym_n, mode, target, sign_pairs)
+           _legs_sig(out_indices), sym.sym_n, mode, target, sign_pairs, out_perm)
 
     def build():
         lib = _lib.load()
@@ -80,11 +86,19 @@ def pairwise_plan(A, B, a_kfirst: int, b_kfirst: int, nc: int, out_indices, mode
         legs = _lib.LegArray([(*ix.sectors(), int(ix.flow)) for ix in out_indices])
         ptr = _lib.C.c_void_p()
         has_t = target is not None
-        sp = np.asarray(sign_pairs, np.int32).reshape(-1)
-        rc = lib.tnx_plan_pairwise_graded(_lib.C.byref(ta.s), _lib.C.byref(tb.s), a_kfirst, b_kfirst, nc, legs.arr,
-                                          sym.sym_n, mode, int(has_t), int(target) if has_t else 0,
-                                          len(sign_pairs), _lib.i32p(sp) if len(sp) else None,
-                                          _lib.stream_handle(), _lib.C.byref(ptr))
+        if out_perm is not None:
+            if sign_pairs:
+                raise ValueError("a permuted output store is not available with fermionic signs")
+            pm = np.asarray(out_perm, np.int32)
+            rc = lib.tnx_plan_pairwise_permuted(_lib.C.byref(ta.s), _lib.C.byref(tb.s), a_kfirst, b_kfirst, nc,
+                                                legs.arr, sym.sym_n, mode, int(has_t), int(target) if has_t else 0,
+                                                _lib.i32p(pm), _lib.stream_handle(), _lib.C.byref(ptr))
+        else:
+            sp = np.asarray(sign_pairs, np.int32).reshape(-1)
+            rc = lib.tnx_plan_pairwise_graded(_lib.C.byref(ta.s), _lib.C.byref(tb.s), a_kfirst, b_kfirst, nc,
+                                              legs.arr, sym.sym_n, mode, int(has_t), int(target) if has_t else 0,
+                                              len(sign_pairs), _lib.i32p(sp) if len(sp) else None,
+                                              _lib.stream_handle(), _lib.C.byref(ptr))
         _lib.check(rc)
         plan = _lib.Plan(ptr)
         keys, shapes, offs, tot = plan.output()

@@ -239,17 +239,55 @@ __device__ __forceinline__ void store_partial(double *__restrict__ W, int row0, 
   }
 }
 
+// Offset of index x of a permuted output side: x decomposed over n legs (the
+// last fastest), digit times stride summed.  x past the side's extent gives an
+// offset that is never used (the element is not stored).
+__device__ __forceinline__ int64_t store_offset(int32_t x, int n, const int32_t *dim, const int64_t *str) {
+  int64_t off = 0;
+  uint32_t v = (uint32_t)x;
+#pragma unroll
+  for (int l = STORE_LEGS - 1; l >= 0; --l) {
+    if (l >= n) continue;
+    const uint32_t d = (uint32_t)dim[l];
+    off += (int64_t)(v % d) * str[l];
+    v /= d;
+  }
+  return off;
+}
+
+// One 16x16 accumulator into a permuted output block: the element of tile row
+// r and tile column c goes to c_off + roff[r] + coff[c] (tables of the tile).
+template <bool F64LAYOUT, class V>
+__device__ __forceinline__ void store_acc_perm(double *__restrict__ C, const GemmOut &o, const int64_t *roff,
+                                               const int64_t *coff, int m0, int n0, int lr0, int lc0, int lane,
+                                               const V &acc, bool accumulate) {
+  const int lc = lc0 + (lane & 15);
+  if (n0 + lc >= o.N) return;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int lr = lr0 + (F64LAYOUT ? ((lane >> 4) + 4 * r) : (((lane >> 4) << 2) + r));
+    if (m0 + lr < o.M) {
+      double *dst = C + o.c_off + roff[lr] + coff[lc];
+      const double v = (double)acc[r];
+      *dst = accumulate ? *dst + v : v;
+    }
+  }
+}
+
 // COMPUTE: 0 = FP64 MFMA, 1 = FP32 MFMA, 2 = split FP32 (3 products, FP64 across K chunks).
 // BM x BN workgroup tile, 4 waves in 2 x 2, each wave (BM/2) x (BN/2) as
 // (BM/32) x (BN/32) MFMA 16x16 blocks.
 // At least two waves per SIMD (the 128-wide tiles held to 256 registers),
 // except split FP32 at 128x128, whose hi/lo operands would then spill.
-template <int ALAY, int COMPUTE, int BM, int BN, bool SIGNED>
+// PERM: outputs may carry a GemmStore (permuted output legs); the other
+// instantiations skip that test.
+template <int ALAY, int COMPUTE, int BM, int BN, bool SIGNED, bool PERM>
 __global__ __launch_bounds__(GEMM_THREADS) __attribute__((amdgpu_waves_per_eu(
     (COMPUTE == 2 && BM * BN > 8192) ? 1 : 2))) void grouped_gemm_kernel(
     const GemmTask *__restrict__ tasks, const GemmOut *__restrict__ outs,
     const GemmPair *__restrict__ pairs, const double *__restrict__ A, const double *__restrict__ B,
-    double *__restrict__ C, double *__restrict__ ws, int accumulate, const int32_t *__restrict__ stop) {
+    double *__restrict__ C, double *__restrict__ ws, int accumulate, const int32_t *__restrict__ stop,
+    const GemmStore *__restrict__ stores) {
   constexpr int NT = (COMPUTE == 2) ? 2 : 1;  // hi/lo planes for split
   constexpr int AP = BM + 1, BP = BN + 1;
   constexpr int TI = BM / 32, TJ = BN / 32;
@@ -259,6 +297,7 @@ __global__ __launch_bounds__(GEMM_THREADS) __attribute__((amdgpu_waves_per_eu(
   // to the other stage, so each K-tile needs a single barrier.
   __shared__ T As[2][NT][GEMM_BK * AP];
   __shared__ T Bs[2][NT][GEMM_BK * BP];
+  __shared__ int64_t tbl[PERM ? BM + BN : 1];  // row / column offsets of a permuted output tile
 
   // a device-side stop flag (a Lanczos solve that has ended): no work
   if (stop && *stop) return;
@@ -489,6 +528,24 @@ __global__ __launch_bounds__(GEMM_THREADS) __attribute__((amdgpu_waves_per_eu(
     return;
   }
   const bool acc_flag = accumulate != 0;
+  if (PERM && stores[t.out].nr > 0) {  // uniform over the workgroup
+    const GemmStore st = stores[t.out];
+    if (tid < BM) tbl[tid] = store_offset(t.m0 + tid, st.nr, st.rdim, st.rstr);
+    else if (tid < BM + BN) tbl[tid] = store_offset(t.n0 + tid - BM, st.nc, st.cdim, st.cstr);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+      for (int j = 0; j < TJ; ++j) {
+        if (COMPUTE == 1)
+          store_acc_perm<false>(C, o, tbl, tbl + BM, t.m0, t.n0, wm + i * 16, wn + j * 16, lane, facc[i][j],
+                                acc_flag);
+        else
+          store_acc_perm<COMPUTE == 0>(C, o, tbl, tbl + BM, t.m0, t.n0, wm + i * 16, wn + j * 16, lane, dacc[i][j],
+                                       acc_flag);
+      }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < TI; ++i)
 #pragma unroll
@@ -505,15 +562,24 @@ __global__ __launch_bounds__(GEMM_THREADS) __attribute__((amdgpu_waves_per_eu(
 // Sum the partial tiles of each split tile in slot order (deterministic) into C.
 // Each thread owns BM*BN/256 elements 256 apart (coalesced) and loads one
 // partial of all of them before adding, so its loads are in flight together.
-template <int BM, int BN>
+template <int BM, int BN, bool PERM>
 __global__ __launch_bounds__(256) void split_reduce_kernel(const SplitTile *__restrict__ red,
                                                            const GemmOut *__restrict__ outs,
                                                            const double *__restrict__ ws, double *__restrict__ C,
-                                                           int accumulate, const int32_t *__restrict__ stop) {
+                                                           int accumulate, const int32_t *__restrict__ stop,
+                                                           const GemmStore *__restrict__ stores) {
   if (stop && *stop) return;
   constexpr int PER = BM * BN / 256;
+  __shared__ int64_t tbl[PERM ? BM + BN : 1];
   const SplitTile r = red[blockIdx.x];
   const GemmOut o = outs[r.out];
+  const bool perm = PERM && stores[r.out].nr > 0;  // uniform over the workgroup
+  if (perm) {
+    const GemmStore st = stores[r.out];
+    if (threadIdx.x < BM) tbl[threadIdx.x] = store_offset(r.m0 + threadIdx.x, st.nr, st.rdim, st.rstr);
+    else if (threadIdx.x < BM + BN) tbl[threadIdx.x] = store_offset(r.n0 + threadIdx.x - BM, st.nc, st.cdim, st.cstr);
+    __syncthreads();
+  }
   const double *w = ws + (int64_t)r.slot0 * (BM * BN) + threadIdx.x;
   double v[PER];
 #pragma unroll
@@ -530,62 +596,71 @@ __global__ __launch_bounds__(256) void split_reduce_kernel(const SplitTile *__re
     const int e = threadIdx.x + i * 256;
     const int row = r.m0 + e / BN, col = r.n0 + e % BN;
     if (row >= o.M || col >= o.N) continue;
-    double *dst = C + o.c_off + (int64_t)row * o.ldc + col;
+    double *dst = perm ? C + o.c_off + tbl[e / BN] + tbl[BM + e % BN] : C + o.c_off + (int64_t)row * o.ldc + col;
     *dst = accumulate ? *dst + v[i] : v[i];
   }
 }
 
-template <int ALAY, int BM, int BN, bool SIGNED>
+template <int ALAY, int BM, int BN, bool SIGNED, bool PERM>
 static void launch_tile(int compute, dim3 grid, hipStream_t s, const GemmTask *tasks, const GemmOut *outs,
                         const GemmPair *pairs, const double *A, const double *B, double *C, double *ws, int acc,
-                        const int32_t *stop) {
+                        const int32_t *stop, const GemmStore *stores) {
   switch (compute) {
     case TNX_COMPUTE_F64:
-      hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, 0, BM, BN, SIGNED>), grid, dim3(GEMM_THREADS), 0, s, tasks, outs, pairs,
-                         A, B, C, ws, acc, stop);
+      hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, 0, BM, BN, SIGNED, PERM>), grid, dim3(GEMM_THREADS), 0, s, tasks,
+                         outs, pairs, A, B, C, ws, acc, stop, stores);
       break;
     case TNX_COMPUTE_F32:
-      hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, 1, BM, BN, SIGNED>), grid, dim3(GEMM_THREADS), 0, s, tasks, outs, pairs,
-                         A, B, C, ws, acc, stop);
+      hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, 1, BM, BN, SIGNED, PERM>), grid, dim3(GEMM_THREADS), 0, s, tasks,
+                         outs, pairs, A, B, C, ws, acc, stop, stores);
       break;
     case TNX_COMPUTE_SPLIT_F32:
-      hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, 2, BM, BN, SIGNED>), grid, dim3(GEMM_THREADS), 0, s, tasks, outs, pairs,
-                         A, B, C, ws, acc, stop);
+      hipLaunchKernelGGL((grouped_gemm_kernel<ALAY, 2, BM, BN, SIGNED, PERM>), grid, dim3(GEMM_THREADS), 0, s, tasks,
+                         outs, pairs, A, B, C, ws, acc, stop, stores);
       break;
     default:
       throw Error{TNX_ERR_DTYPE, "unknown compute mode " + std::to_string(compute)};
   }
 }
 
-template <int ALAY, bool SIGNED>
+template <int ALAY, bool SIGNED, bool PERM>
 static void launch_shape(int shape, int compute, dim3 grid, hipStream_t s, const GemmTask *tasks,
                          const GemmOut *outs, const GemmPair *pairs, const double *A, const double *B, double *C,
-                         double *ws, int acc, const int32_t *stop) {
+                         double *ws, int acc, const int32_t *stop, const GemmStore *stores) {
   switch (shape) {
-    case 0: launch_tile<ALAY, 64, 64, SIGNED>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop); break;
-    case 1: launch_tile<ALAY, 128, 64, SIGNED>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop); break;
-    case 2: launch_tile<ALAY, 64, 128, SIGNED>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop); break;
-    case 3: launch_tile<ALAY, 128, 128, SIGNED>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop); break;
+    case 0: launch_tile<ALAY, 64, 64, SIGNED, PERM>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop, stores); break;
+    case 1: launch_tile<ALAY, 128, 64, SIGNED, PERM>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop, stores); break;
+    case 2: launch_tile<ALAY, 64, 128, SIGNED, PERM>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop, stores); break;
+    case 3: launch_tile<ALAY, 128, 128, SIGNED, PERM>(compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop, stores); break;
     default: throw Error{TNX_ERR_ARG, "unknown tile shape " + std::to_string(shape)};
   }
 }
 
+// stores != nullptr: the program's outputs carry GemmStore descriptors
+// (permuted output legs; bosonic programs only).
 static void launch_any(int a_lay, int b_lay, bool signs, int shape, int compute, int n_tasks, hipStream_t s,
                        const GemmTask *tasks, const GemmOut *outs, const GemmPair *pairs, const double *A,
-                       const double *B, double *C, double *ws, int acc, const int32_t *stop) {
+                       const double *B, double *C, double *ws, int acc, const int32_t *stop,
+                       const GemmStore *stores = nullptr) {
   if (n_tasks <= 0) return;
   TNX_REQUIRE(b_lay == LAY_KN, TNX_ERR_ARG, "B operands must be K x N row-major");
+  TNX_REQUIRE(!(signs && stores), TNX_ERR_ARG, "permuted output stores are not available with fermionic signs");
   dim3 grid(n_tasks);
-  if (a_lay == LAY_MK) {
-    if (signs)
-      launch_shape<LAY_MK, true>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop);
+  if (stores) {
+    if (a_lay == LAY_MK)
+      launch_shape<LAY_MK, false, true>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop, stores);
     else
-      launch_shape<LAY_MK, false>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop);
+      launch_shape<LAY_KM, false, true>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop, stores);
+  } else if (a_lay == LAY_MK) {
+    if (signs)
+      launch_shape<LAY_MK, true, false>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop, nullptr);
+    else
+      launch_shape<LAY_MK, false, false>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop, nullptr);
   } else {
     if (signs)
-      launch_shape<LAY_KM, true>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop);
+      launch_shape<LAY_KM, true, false>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop, nullptr);
     else
-      launch_shape<LAY_KM, false>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop);
+      launch_shape<LAY_KM, false, false>(shape, compute, grid, s, tasks, outs, pairs, A, B, C, ws, acc, stop, nullptr);
   }
   TNX_CHECK_HIP(hipGetLastError());
 }
@@ -593,27 +668,37 @@ static void launch_any(int a_lay, int b_lay, bool signs, int shape, int compute,
 void launch_grouped_gemm(const GemmProgram &prog, const double *A, const double *B, double *C, int compute,
                          bool accumulate, hipStream_t s, bool reduce, const int32_t *stop) {
   const int acc = accumulate ? 1 : 0;
+  const GemmStore *stores = prog.permuted ? prog.stores.p : nullptr;
   if (!prog.tiled) {
     launch_any(prog.a_lay, prog.b_lay, prog.signs, 0, compute, prog.n_tasks, s, prog.tasks.p, prog.outs.p, prog.pairs.p, A, B,
-               C, nullptr, acc, stop);
+               C, nullptr, acc, stop, stores);
     return;
   }
   launch_any(prog.a_lay, prog.b_lay, prog.signs, prog.cfg.shape, compute, prog.n_vtasks, s, prog.vtasks.p, prog.outs.p,
-             prog.pairs.p, A, B, C, prog.ws.p, acc, stop);
+             prog.pairs.p, A, B, C, prog.ws.p, acc, stop, stores);
   if (reduce) launch_split_reduce(prog, C, accumulate, s, stop);
+}
+
+template <bool PERM>
+static void launch_reduce(const GemmProgram &prog, double *C, int acc, hipStream_t s, const int32_t *stop) {
+  const dim3 g(prog.n_red), b(256);
+  const GemmStore *st = PERM ? prog.stores.p : nullptr;
+  switch (prog.cfg.shape) {
+    case 0: hipLaunchKernelGGL((split_reduce_kernel<64, 64, PERM>), g, b, 0, s, prog.red.p, prog.outs.p, prog.ws.p, C, acc, stop, st); break;
+    case 1: hipLaunchKernelGGL((split_reduce_kernel<128, 64, PERM>), g, b, 0, s, prog.red.p, prog.outs.p, prog.ws.p, C, acc, stop, st); break;
+    case 2: hipLaunchKernelGGL((split_reduce_kernel<64, 128, PERM>), g, b, 0, s, prog.red.p, prog.outs.p, prog.ws.p, C, acc, stop, st); break;
+    default: hipLaunchKernelGGL((split_reduce_kernel<128, 128, PERM>), g, b, 0, s, prog.red.p, prog.outs.p, prog.ws.p, C, acc, stop, st); break;
+  }
+  TNX_CHECK_HIP(hipGetLastError());
 }
 
 void launch_split_reduce(const GemmProgram &prog, double *C, bool accumulate, hipStream_t s, const int32_t *stop) {
   const int acc = accumulate ? 1 : 0;
   if (prog.tiled && prog.n_red > 0) {
-    const dim3 g(prog.n_red), b(256);
-    switch (prog.cfg.shape) {
-      case 0: hipLaunchKernelGGL((split_reduce_kernel<64, 64>), g, b, 0, s, prog.red.p, prog.outs.p, prog.ws.p, C, acc, stop); break;
-      case 1: hipLaunchKernelGGL((split_reduce_kernel<128, 64>), g, b, 0, s, prog.red.p, prog.outs.p, prog.ws.p, C, acc, stop); break;
-      case 2: hipLaunchKernelGGL((split_reduce_kernel<64, 128>), g, b, 0, s, prog.red.p, prog.outs.p, prog.ws.p, C, acc, stop); break;
-      default: hipLaunchKernelGGL((split_reduce_kernel<128, 128>), g, b, 0, s, prog.red.p, prog.outs.p, prog.ws.p, C, acc, stop); break;
-    }
-    TNX_CHECK_HIP(hipGetLastError());
+    if (prog.permuted)
+      launch_reduce<true>(prog, C, acc, s, stop);
+    else
+      launch_reduce<false>(prog, C, acc, s, stop);
   }
 }
 

@@ -156,17 +156,35 @@ constexpr int GEMM_BN = 64;
 constexpr int GEMM_BK = 16;
 constexpr int GEMM_THREADS = 256;
 
+// Where an output block's elements go when the output legs are stored in
+// another order than [A free legs..., B free legs...] (a contraction whose
+// requested output order the GEMM writes directly): element (row, col) of the
+// block's M x N product lands at c_off + rowoff(row) + coloff(col), where
+// rowoff decomposes row over nr row legs (last fastest) and sums digit x
+// stride, and likewise for columns.  nr == 0: the plain row-major store
+// c_off + row * ldc + col.  Legs contiguous in both orders are merged, so
+// three per side cover the output permutations of rank-6 contractions.
+constexpr int STORE_LEGS = 3;
+struct GemmStore {
+  int32_t nr, nc;
+  int32_t rdim[STORE_LEGS], cdim[STORE_LEGS];
+  int64_t rstr[STORE_LEGS], cstr[STORE_LEGS];
+};
+
 // Device-resident grouped-GEMM program: the work lists of one launch.
 struct GemmProgram {
   int a_lay = LAY_MK, b_lay = LAY_KN;
   bool signs = false;  // some pair carries a fermionic sign (set when finalized)
+  bool permuted = false;  // some output uses a GemmStore with nr > 0 (stores[] is then one per output)
   DevBuf<GemmPair> pairs;
   DevBuf<GemmOut> outs;
+  DevBuf<GemmStore> stores;
   DevBuf<GemmTask> tasks;
   int32_t n_pairs = 0, n_outs = 0, n_tasks = 0;
   // host mirrors (used by the per-block path, tiling and stats)
   std::vector<GemmPair> h_pairs;
   std::vector<GemmOut> h_outs;
+  std::vector<GemmStore> h_stores;
   double flops = 0.0;
   // tiled variant (set_tiling): used by launch_grouped_gemm when `tiled`
   bool tiled = false;

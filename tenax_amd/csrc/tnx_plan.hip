@@ -922,6 +922,106 @@ static void build_pairwise(PairwisePlan &P, const tnx_blocks *a, const tnx_block
   if (validation_enabled()) validate_pairwise(P, s);
 }
 
+// Store the output legs of a built pairwise plan in the order out_perm (output
+// leg j = natural leg out_perm[j]; natural = [A free legs..., B free legs...]):
+// the output blocks are re-keyed, re-sorted and re-laid out in that order, and
+// every output of the program gets a GemmStore so the GEMM writes each element
+// straight to its place in the permuted block (no separate transpose).  Sides
+// (rows = A free legs, columns = B free legs) merge legs that stay adjacent;
+// more than STORE_LEGS legs left on a side is refused (TNX_ERR_ARG).
+static void apply_output_perm(PairwisePlan &P, int nfa, const int32_t *perm, hipStream_t s) {
+  const int r = P.out_rank;
+  TNX_REQUIRE(perm, TNX_ERR_ARG, "null output permutation");
+  TNX_REQUIRE(nfa >= 0 && nfa <= r, TNX_ERR_ARG, "bad free-leg split");
+  std::vector<int> inv(r, -1);
+  bool identity = true;
+  for (int j = 0; j < r; ++j) {
+    TNX_REQUIRE(perm[j] >= 0 && perm[j] < r && inv[perm[j]] < 0, TNX_ERR_ARG, "out_perm is not a permutation");
+    inv[perm[j]] = j;
+    identity = identity && perm[j] == j;
+  }
+  if (identity || P.prog.n_outs == 0) return;
+  TNX_REQUIRE(!P.prog.signs, TNX_ERR_ARG, "permuted output stores are not available with fermionic signs");
+  const int64_t nout = P.out_nblocks;
+  TNX_REQUIRE((int64_t)P.prog.h_outs.size() == nout, TNX_ERR_ARG, "plan has no host mirror of its outputs");
+  // permuted keys and shapes, blocks sorted by permuted key
+  std::vector<int32_t> pk(nout * r), ps(nout * r);
+  for (int64_t o = 0; o < nout; ++o)
+    for (int j = 0; j < r; ++j) {
+      pk[o * r + j] = P.out_keys[o * r + perm[j]];
+      ps[o * r + j] = P.out_shapes[o * r + perm[j]];
+    }
+  std::vector<int64_t> order(nout);
+  for (int64_t o = 0; o < nout; ++o) order[o] = o;
+  std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
+    return std::lexicographical_compare(&pk[x * r], &pk[x * r] + r, &pk[y * r], &pk[y * r] + r);
+  });
+  std::vector<int64_t> noff(nout);
+  std::vector<int32_t> keys2(nout * r), shapes2(nout * r);
+  std::vector<int64_t> offs2(nout);
+  int64_t at = 0;
+  for (int64_t q = 0; q < nout; ++q) {
+    const int64_t o = order[q];
+    int64_t sz = 1;
+    for (int j = 0; j < r; ++j) sz *= ps[o * r + j];
+    noff[o] = at;
+    offs2[q] = at;
+    std::copy(&pk[o * r], &pk[o * r] + r, &keys2[q * r]);
+    std::copy(&ps[o * r], &ps[o * r] + r, &shapes2[q * r]);
+    at += sz;
+  }
+  // the stores: natural leg i sits at permuted position inv[i]
+  auto side = [&](int64_t o, int l0, int l1, int32_t *dim, int64_t *str) {
+    std::vector<int32_t> d;
+    std::vector<int64_t> st;
+    for (int i = l0; i < l1; ++i) {
+      int64_t stride = 1;
+      for (int j = inv[i] + 1; j < r; ++j) stride *= ps[o * r + j];
+      const int32_t di = P.out_shapes[o * r + i];
+      if (di == 1) continue;
+      if (!d.empty() && st.back() == stride * di) {  // contiguous with the leg before: merge
+        d.back() *= di;
+        st.back() = stride;
+        continue;
+      }
+      d.push_back(di);
+      st.push_back(stride);
+    }
+    if (d.empty()) {
+      d.push_back(1);
+      st.push_back(0);
+    }
+    TNX_REQUIRE((int)d.size() <= STORE_LEGS, TNX_ERR_ARG,
+                "output permutation needs more than 3 row or column legs after merging");
+    for (size_t k = 0; k < d.size(); ++k) {
+      dim[k] = d[k];
+      str[k] = st[k];
+    }
+    return (int32_t)d.size();
+  };
+  P.prog.h_stores.assign(nout, GemmStore{});
+  for (int64_t o = 0; o < nout; ++o) {
+    GemmStore g{};
+    g.nr = side(o, 0, nfa, g.rdim, g.rstr);
+    g.nc = side(o, nfa, r, g.cdim, g.cstr);
+    GemmOut &h = P.prog.h_outs[o];
+    int64_t last = 0;
+    for (int k = 0; k < g.nr; ++k) last += (int64_t)(g.rdim[k] - 1) * g.rstr[k];
+    for (int k = 0; k < g.nc; ++k) last += (int64_t)(g.cdim[k] - 1) * g.cstr[k];
+    TNX_REQUIRE((int64_t)h.M * h.N == 0 || last == (int64_t)h.M * h.N - 1, TNX_ERR_ARG,
+                "permuted store does not cover its block");
+    h.c_off = noff[o];
+    P.prog.h_stores[o] = g;
+  }
+  P.prog.outs.upload(P.prog.h_outs.data(), nout, s);
+  P.prog.stores.upload(P.prog.h_stores.data(), nout, s);
+  TNX_CHECK_HIP(hipStreamSynchronize(s));
+  P.prog.permuted = true;
+  P.out_keys.swap(keys2);
+  P.out_shapes.swap(shapes2);
+  P.out_offsets.swap(offs2);
+}
+
 // Upload a host-built program and let the device build its tile list.
 static void finalize_host_program(GemmProgram &g, int a_lay, const std::vector<GemmPair> &pairs,
                                   const std::vector<GemmOut> &outs, hipStream_t s) {
@@ -1454,6 +1554,21 @@ int tnx_plan_pairwise(const tnx_blocks *a, const tnx_blocks *b, int32_t a_kfirst
   });
 }
 
+int tnx_plan_pairwise_permuted(const tnx_blocks *a, const tnx_blocks *b, int32_t a_kfirst, int32_t b_kfirst,
+                               int32_t n_contracted, const tnx_leg *out_legs, int32_t sym_n, int32_t mode,
+                               int32_t has_target, int32_t target, const int32_t *out_perm, void *stream,
+                               tnx_plan **plan) {
+  return guarded([&] {
+    TNX_REQUIRE(plan, TNX_ERR_ARG, "null plan pointer");
+    std::unique_ptr<PairwisePlan> P(new PairwisePlan());
+    build_pairwise(*P, a, b, a_kfirst, b_kfirst, n_contracted, out_legs, sym_n, mode, has_target != 0, target,
+                   (hipStream_t)stream);
+    apply_output_perm(*P, a->rank - n_contracted, out_perm, (hipStream_t)stream);
+    if (validation_enabled()) validate_pairwise(*P, (hipStream_t)stream);
+    *plan = P.release();
+  });
+}
+
 int tnx_plan_pairwise_graded(const tnx_blocks *a, const tnx_blocks *b, int32_t a_kfirst, int32_t b_kfirst,
                              int32_t n_contracted, const tnx_leg *out_legs, int32_t sym_n, int32_t mode,
                              int32_t has_target, int32_t target, int32_t n_sign_pairs, const int32_t *sign_pairs,
@@ -1573,6 +1688,7 @@ int tnx_contract_execute_perblock(const tnx_plan *plan, const double *a, const d
   return guarded([&] {
     TNX_REQUIRE(plan && plan->kind == PlanKind::Pairwise, TNX_ERR_ARG, "not a pairwise plan");
     PairwisePlan &P = const_cast<PairwisePlan &>(static_cast<const PairwisePlan &>(*plan));
+    TNX_REQUIRE(!P.prog.permuted, TNX_ERR_ARG, "the per-block path writes the natural output order only");
     hipStream_t s = (hipStream_t)stream;
     ensure_perblock(P, s);
     TNX_REQUIRE(a && b && c || P.prog.n_pairs == 0, TNX_ERR_ARG, "null buffer");
@@ -1588,6 +1704,7 @@ int tnx_contract_vjp(const tnx_plan *plan, const double *g_c, const double *a, c
     TNX_REQUIRE(plan && plan->kind == PlanKind::Pairwise, TNX_ERR_ARG, "not a pairwise plan");
     PairwisePlan &P = const_cast<PairwisePlan &>(static_cast<const PairwisePlan &>(*plan));
     TNX_REQUIRE(compute >= 0 && compute <= 2, TNX_ERR_DTYPE, "unknown compute mode");
+    TNX_REQUIRE(!P.prog.permuted, TNX_ERR_ARG, "no adjoint for a plan with a permuted output order");
     hipStream_t s = (hipStream_t)stream;
     ensure_adjoint(P, s);
     if (g_b) {
@@ -1683,8 +1800,17 @@ int tnx_batch_pairwise(const tnx_plan *const *plans, int32_t n, const double *co
     G.b_lay = LAY_KN;
     std::vector<GemmPair> pairs;
     std::vector<GemmOut> outs;
+    std::vector<GemmStore> stores;
+    bool permuted = false;
+    for (int i : live) permuted = permuted || static_cast<const PairwisePlan &>(*plans[i]).prog.permuted;
     for (int i : live) {
       const GemmProgram &g = static_cast<const PairwisePlan &>(*plans[i]).prog;
+      if (permuted) {
+        if (g.permuted)
+          stores.insert(stores.end(), g.h_stores.begin(), g.h_stores.end());
+        else
+          stores.resize(stores.size() + g.h_outs.size(), GemmStore{});  // nr == 0: plain row-major store
+      }
       const int64_t da = (int64_t)(((uintptr_t)a[i] - amin) / 8), db = (int64_t)(((uintptr_t)b[i] - bmin) / 8),
                     dc = (int64_t)(((uintptr_t)c[i] - cmin) / 8);
       const int32_t p0 = (int32_t)pairs.size();
@@ -1706,6 +1832,12 @@ int tnx_batch_pairwise(const tnx_plan *const *plans, int32_t n, const double *co
     G.n_outs = (int32_t)outs.size();
     G.pairs.upload(pairs.data(), pairs.size(), s);
     G.outs.upload(outs.data(), outs.size(), s);
+    if (permuted) {
+      TNX_REQUIRE(stores.size() == outs.size(), TNX_ERR_ARG, "store list does not match the outputs");
+      G.h_stores = stores;
+      G.stores.upload(G.h_stores.data(), G.h_stores.size(), s);
+      G.permuted = true;
+    }
     finalize_program(G, s);
     B->A = (const double *)amin;
     B->B = (const double *)bmin;

@@ -89,3 +89,16 @@ def test_label_subscripts_sorted_by_str():
     # sorted by str: "10" < "a" < "b" -> 10:a, a:b, b:c
     assert subs == "ca,ab->cb"
     assert [ix.label for ix in out] == ["b", "a"]
+
+
+def test_fused_output_order_run_count():
+    """The GEMM writes a requested output order itself only when the A free
+    legs (rows) and the B free legs (columns) each fall into at most three
+    runs of legs that stay adjacent in that order."""
+    from tenax_amd.contraction.contractor import _fused_order_fits
+
+    assert _fused_order_fits((0, 1, 2, 3), 2)          # natural order: one run a side
+    assert _fused_order_fits((3, 2, 1, 0), 2)          # two legs a side, reversed: two runs each
+    assert _fused_order_fits((2, 0, 3, 1, 4, 5), 3)    # rows 0,1,2 at 1,3,0 -> 3 runs
+    assert not _fused_order_fits((3, 2, 4, 1, 0), 4)   # rows a,b,c,d reversed: 4 runs
+    assert _fused_order_fits((4, 0, 1, 2, 3), 4)       # rows contiguous, columns first

@@ -455,3 +455,116 @@ def test_split_over_workspace_cap_is_refused_and_skipped(gpu, monkeypatch):
     contractor._execute_pair(plan, A, B, y)
     assert float((y - ref).abs().max() / ref.abs().max()) <= 1e-13
     autotune.clear()
+
+
+FUSED_ORDERS = [("s", "qq", "q", "pp", "p", "l"), ("q", "l", "qq", "p", "s", "pp"), ("p", "l", "s", "pp", "q", "qq"),
+                ("l", "q", "qq", "s", "p", "pp")]
+
+
+def _natural(fn):
+    from tenax_amd.contraction import contractor
+
+    old = contractor.FUSED_OUTPUT_ORDER
+    contractor.FUSED_OUTPUT_ORDER = False
+    try:
+        return fn()
+    finally:
+        contractor.FUSED_OUTPUT_ORDER = old
+
+
+@pytest.mark.parametrize("order", FUSED_ORDERS)
+def test_fused_output_order_matches_oracle_and_transpose(gpu, c2_pair, order):
+    """contract(..., output_labels=order) with the GEMM writing the requested
+    leg order itself (permuted-store plan): the oracle's blocks for that order,
+    and bit for bit the natural result transposed, under every tiling."""
+    from tenax_amd import contract
+    from tenax_amd.contraction import contractor
+
+    oA, oB, _ = c2_pair
+    A, B = from_oracle(oA), from_oracle(oB)
+    sub = {"l": "l", "p": "p", "pp": "m", "q": "q", "qq": "n", "s": "s"}
+    oC, _ = ocn.contract_symmetric([oA, oB], "lpmr,rqns->" + "".join(sub[x] for x in order))
+    assert contractor.FUSED_OUTPUT_ORDER
+    C = contract(A, B, output_labels=order)
+    plan = _last_pair_plan()
+    assert C.labels() == order
+    assert_same_structure(C, oC)
+    assert max_rel_err(C, oC) < 1e-13
+    R = _natural(lambda: contract(A, B, output_labels=order))
+    assert np.array_equal(C.keys, R.keys) and np.array_equal(C.shapes, R.shapes)
+    assert np.array_equal(C.offsets, R.offsets)
+    assert torch.equal(C.data, R.data)
+    for cfg in [(sh, kc, xc) for sh in (0, 1, 2, 3) for kc in (0, 1, -1, -2) for xc in (0, 1)]:
+        try:
+            plan.set_tiling(0, *cfg)
+        except Exception as e:  # a split over the workspace cap is refused, not run
+            assert "workspace" in str(e), cfg
+            continue
+        C2 = contract(A, B, output_labels=order)
+        d = (C2.data - R.data).abs().max().item()
+        assert d <= 1e-13 * R.data.abs().max().item(), cfg
+    plan.set_tiling(0, 0, 0, 0)
+
+
+def test_fused_output_order_in_many_instance_launches(gpu, c2_pair):
+    """contract_many with an output order: the instances' permuted-store plans
+    merged into one launch, equal bit for bit to transposing each natural
+    result; mixing a natural-order instance into the batch keeps it plain."""
+    from tenax_amd.contraction import contract_many
+
+    oA, oB, _ = c2_pair
+    A, B = from_oracle(oA), from_oracle(oB)
+    A2 = A._with_data(A.data * 0.5 + 1.0)
+    order = FUSED_ORDERS[1]
+    got = contract_many([(A, B), (A2, B)], output_labels=order)
+    ref = _natural(lambda: contract_many([(A, B), (A2, B)], output_labels=order))
+    for g, r in zip(got, ref):
+        assert g.labels() == order and np.array_equal(g.keys, r.keys)
+        assert torch.equal(g.data, r.data)
+    nat = contract_many([(A, B), (A2, B)])
+    from tenax_amd import contract
+
+    for g, (a, b) in zip(nat, [(A, B), (A2, B)]):
+        assert torch.equal(g.data, contract(a, b).data)
+
+
+def test_permuted_store_limits(gpu):
+    """A side needing more than three strides is refused by the planner and
+    contract() transposes the natural result instead; a permuted-store plan
+    has no adjoint and no per-block path."""
+    from tenax_amd import _lib, contract, per_block_execution
+    from tenax_amd.contraction import contractor, plans
+
+    rng = np.random.default_rng(5)
+    U1 = oc.U1()
+    ch = lambda: rng.integers(-1, 2, size=4)
+    la = tuple(oc.Leg(U1, ch(), oc.IN if i % 2 else oc.OUT, n) for i, n in enumerate("abcde"))
+    lb = (oc.Leg(U1, la[4].charges, oc.IN if la[4].flow == oc.OUT else oc.OUT, "e"),
+          oc.Leg(U1, ch(), oc.IN, "f"))
+    oA = oc.SymT.random_normal(la, (1, 0))
+    oB = oc.SymT.random_normal(lb, (1, 1))
+    A, B = from_oracle(oA), from_oracle(oB)
+    order = ("d", "c", "f", "b", "a")  # A's free legs a, b, c, d reversed: four runs
+    oC, _ = ocn.contract_symmetric([oA, oB], "abcde,ef->dcfba")
+    C = contract(A, B, output_labels=order)
+    assert_same_structure(C, oC)
+    assert max_rel_err(C, oC) < 1e-13
+    assert not contractor._fused_order_fits((3, 2, 4, 1, 0), 4)
+    nat = tuple(A.indices[:4]) + (B.indices[1],)
+    with pytest.raises(_lib.TnxError, match="more than 3"):
+        plans.pairwise_plan(A, B, 0, 1, 1, nat, _lib.MODE_PAIRWISE_FILTERED, None, out_perm=(3, 2, 4, 1, 0))
+    # a fitting permutation: no adjoint, no per-block execution
+    plan = plans.pairwise_plan(A, B, 0, 1, 1, nat, _lib.MODE_PAIRWISE_FILTERED, None, out_perm=(4, 0, 1, 2, 3))[0]
+    lib = _lib.load()
+    g = torch.zeros(1, dtype=torch.float64, device="cuda")
+    with pytest.raises(_lib.TnxError, match="no adjoint"):
+        _lib.check(lib.tnx_contract_vjp(plan.ptr, _lib.dptr(g), _lib.dptr(A.data), _lib.dptr(B.data),
+                                        _lib.dptr(A.data.clone()), None, 0, _lib.stream_handle()))
+    with pytest.raises(_lib.TnxError, match="per-block"):
+        _lib.check(lib.tnx_contract_execute_perblock(plan.ptr, _lib.dptr(A.data), _lib.dptr(B.data),
+                                                     _lib.dptr(g), 0, _lib.stream_handle()))
+    with per_block_execution():
+        Cp = contract(A, B, output_labels=("f", "a", "b", "c", "d"))
+    Cg = contract(A, B, output_labels=("f", "a", "b", "c", "d"))
+    assert np.array_equal(Cp.keys, Cg.keys)
+    assert (Cp.data - Cg.data).abs().max().item() <= 1e-12 * Cg.data.abs().max().item()
