@@ -83,8 +83,9 @@ def stage_kernel(prob, stage):
     t = prob.plan.tiling(stage)
     bm, bn = TILE_SHAPES[t["shape"]]
     lay = 1 if stage == 1 else 0  # stage 1 reads L as K x M, stage 3 reads Y as M x K
-    # last template argument: fermionic signs (the Heisenberg chain is bosonic)
-    return f"grouped_gemm_kernel<{lay},0,{bm},{bn},false>", t["n_tasks"], t
+    # last two template arguments: fermionic signs (the Heisenberg chain is
+    # bosonic) and permuted output stores (the matvec writes plain blocks)
+    return f"grouped_gemm_kernel<{lay},0,{bm},{bn},false,false>", t["n_tasks"], t
 
 
 def time_events(fn, reps, warm=0):
