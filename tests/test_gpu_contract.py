@@ -473,13 +473,16 @@ def _natural(fn):
 
 
 @pytest.mark.parametrize("order", FUSED_ORDERS)
-def test_fused_output_order_matches_oracle_and_transpose(gpu, c2_pair, order):
+def test_fused_output_order_matches_oracle_and_transpose(gpu, c2_pair, order, monkeypatch):
     """contract(..., output_labels=order) with the GEMM writing the requested
     leg order itself (permuted-store plan): the oracle's blocks for that order,
-    and bit for bit the natural result transposed, under every tiling."""
+    and bit for bit the natural result transposed, under every tiling (each
+    work list checked against the operand and output block layouts)."""
     from tenax_amd import contract
-    from tenax_amd.contraction import contractor
+    from tenax_amd.contraction import contractor, plans
 
+    monkeypatch.setenv("TNX_VALIDATE", "1")
+    plans.clear_plan_cache()
     oA, oB, _ = c2_pair
     A, B = from_oracle(oA), from_oracle(oB)
     sub = {"l": "l", "p": "p", "pp": "m", "q": "q", "qq": "n", "s": "s"}
