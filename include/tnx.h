@@ -137,7 +137,8 @@ int tnx_plan_free(tnx_plan *plan);
 
 /* Tiling of one grouped-GEMM launch of a plan (no reference counterpart: the
  * reference dispatches one einsum per block pair).  stage: 0 for a pairwise
- * plan; 1 (L.theta) or 3 (.R) for a matvec / environment plan.
+ * plan or a batch of them (tnx_batch_pairwise); 1 (L.theta) or 3 (.R) for a
+ * matvec / environment plan.
  *   shape  0 = 64x64, 1 = 128x64, 2 = 64x128, 3 = 128x128 output tiles per workgroup
  *   kchunk K tiles (of 16) per task: 0 = whole K, -2 = balanced two-wave split
  *          (tiles whole up to the mean work of one workgroup slot, longer ones

@@ -240,6 +240,11 @@ def _is_graded(T) -> bool:
 FUSED_OUTPUT_ORDER = True
 _STORE_LEGS = 3  # STORE_LEGS of the library
 
+# Tiling (shape, kchunk, xcd) of the merged grouped-GEMM launch of
+# contract_many / contract_pairs_batched (tnx_plan_set_tiling on the batch
+# plan); None keeps the default 64x64 whole-K list.
+BATCH_TILING = None
+
 
 def _fused_order_fits(out_axes, nfa: int) -> bool:
     pos = {leg: j for j, leg in enumerate(out_axes)}
@@ -424,6 +429,12 @@ def contract_pairs_batched(jobs) -> list[SymmetricTensor]:
         batch = _lib.C.c_void_p()
         _lib.check(lib.tnx_batch_pairwise(P, n, Aa, Bb, Cc, _lib.stream_handle(), _lib.C.byref(batch)))
         bplan = _lib.Plan(batch)
+        if BATCH_TILING is not None:
+            try:
+                bplan.set_tiling(0, *BATCH_TILING)
+            except _lib.TnxError as e:  # a split over the workspace cap: keep the default list
+                if e.code != _lib.TNX_ERR_CAPACITY:
+                    raise
         if WORK["enabled"]:
             WORK["flops"] += bplan.stats()["flops"]
             WORK["launches"] += 1

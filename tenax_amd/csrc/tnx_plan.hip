@@ -1636,6 +1636,10 @@ static GemmProgram *stage_program(tnx_plan *plan, int stage) {
     TNX_REQUIRE(stage == 0, TNX_ERR_ARG, "a pairwise plan has one GEMM stage (0)");
     return &static_cast<PairwisePlan &>(*plan).prog;
   }
+  if (plan->kind == PlanKind::BatchPairwise) {
+    TNX_REQUIRE(stage == 0, TNX_ERR_ARG, "a batch of pairwise plans has one GEMM stage (0)");
+    return &static_cast<BatchPairwisePlan &>(*plan).prog;
+  }
   TNX_REQUIRE(plan->kind == PlanKind::Matvec || plan->kind == PlanKind::EnvLeft, TNX_ERR_ARG,
               "plan has no grouped-GEMM stage");
   return fused_program(plan, stage);

@@ -571,3 +571,24 @@ def test_permuted_store_limits(gpu):
     Cg = contract(A, B, output_labels=("f", "a", "b", "c", "d"))
     assert np.array_equal(Cp.keys, Cg.keys)
     assert (Cp.data - Cg.data).abs().max().item() <= 1e-12 * Cg.data.abs().max().item()
+
+
+@pytest.mark.parametrize("tiling", [(0, 0, 1), (0, -2, 1), (0, -1, 0), (1, 0, 1), (3, -1, 1)])
+def test_batched_launch_tilings_agree(gpu, c2_pair, tiling, monkeypatch):
+    """A tiling set on the merged launch of contract_many (tnx_plan_set_tiling
+    on the batch plan) gives the default launch's results, with natural and
+    with fused output orders."""
+    from tenax_amd.contraction import contract_many, contractor
+
+    oA, oB, _ = c2_pair
+    A, B = from_oracle(oA), from_oracle(oB)
+    A2 = A._with_data(A.data * 0.5 + 1.0)
+    for order in (None, FUSED_ORDERS[1]):
+        monkeypatch.setattr(contractor, "BATCH_TILING", None)
+        ref = contract_many([(A, B), (A2, B)], output_labels=order)
+        monkeypatch.setattr(contractor, "BATCH_TILING", tiling)
+        got = contract_many([(A, B), (A2, B)], output_labels=order)
+        for g, r in zip(got, ref):
+            assert np.array_equal(g.keys, r.keys)
+            d = (g.data - r.data).abs().max().item()
+            assert d <= 1e-13 * r.data.abs().max().item(), (tiling, order)
