@@ -592,3 +592,47 @@ def test_batched_launch_tilings_agree(gpu, c2_pair, tiling, monkeypatch):
             assert np.array_equal(g.keys, r.keys)
             d = (g.data - r.data).abs().max().item()
             assert d <= 1e-13 * r.data.abs().max().item(), (tiling, order)
+
+
+@pytest.mark.parametrize("sym_n,seed", [(3, 11), (None, 12), (2, 13)])
+def test_fused_output_order_random_structures(gpu, sym_n, seed):
+    """Random Z_n / U(1) operands with a charge target on A, contracted over
+    two legs, every output order of the five free legs that fits the store
+    descriptors: the oracle's blocks for that order and the natural result
+    transposed, bit for bit."""
+    import itertools
+
+    from tenax_amd import contract
+    from tenax_amd.contraction import contractor
+
+    rng = np.random.default_rng(seed)
+    sym = oc.U1() if sym_n is None else oc.Zn(sym_n)
+
+    def ch(d):
+        return rng.integers(-2, 3, size=d) if sym_n is None else rng.integers(0, sym_n, size=d)
+
+    la = tuple(oc.Leg(sym, ch(5), oc.IN if i % 2 else oc.OUT, n) for i, n in enumerate("abcd"))
+    lb = (oc.Leg(sym, la[2].charges, oc.OUT if la[2].flow == oc.IN else oc.IN, "c"),
+          oc.Leg(sym, ch(4), oc.IN, "e"),
+          oc.Leg(sym, la[3].charges, oc.OUT if la[3].flow == oc.IN else oc.IN, "d"),
+          oc.Leg(sym, ch(3), oc.OUT, "f"),
+          oc.Leg(sym, ch(4), oc.IN, "g"))
+    oA = oc.SymT.random_normal(la, (seed, 0), target=1)
+    oB = oc.SymT.random_normal(lb, (seed, 1))
+    A, B = from_oracle(oA), from_oracle(oB)
+    free = ("a", "b", "e", "f", "g")
+    nat_pos = {l: i for i, l in enumerate(free)}
+    tried = 0
+    for order in itertools.permutations(free):
+        if order == free or not contractor._fused_order_fits(tuple(nat_pos[l] for l in order), 2):
+            continue
+        tried += 1
+        if tried % 7:  # every 7th fitting order keeps the test short
+            continue
+        oC, _ = ocn.contract_symmetric([oA, oB], "abcd,cedfg->" + "".join(order))
+        C = contract(A, B, output_labels=order)
+        assert_same_structure(C, oC)
+        assert max_rel_err(C, oC) < 1e-13
+        R = _natural(lambda: contract(A, B, output_labels=order))
+        assert np.array_equal(C.keys, R.keys) and torch.equal(C.data, R.data), order
+    assert tried > 20
