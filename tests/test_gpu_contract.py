@@ -636,3 +636,49 @@ def test_fused_output_order_random_structures(gpu, sym_n, seed):
         R = _natural(lambda: contract(A, B, output_labels=order))
         assert np.array_equal(C.keys, R.keys) and torch.equal(C.data, R.data), order
     assert tried > 20
+
+
+def test_empty_operands_and_results(gpu):
+    """Tensors without a single valid block, and contractions whose operands
+    share no sector, through contract, transpose, add, inner, norm and
+    max_abs: structure as the oracle's (zero blocks, the same legs), zero
+    values, no launch on an empty work list (tensor.py:923-967, 296-319;
+    contractor.py:444-476 iterate over no block combination)."""
+    from tenax_amd import contract, inner
+
+    U1 = oc.U1()
+    rng = np.random.default_rng(5)
+    # no valid block at all: every row charge 5 against column charges in [-1, 1]
+    Ae = (oc.Leg(U1, np.array([5, 5, 5]), oc.IN, "i"), oc.Leg(U1, rng.integers(-1, 2, 4), oc.OUT, "j"))
+    oAe = oc.SymT.random_normal(Ae, (1,), target=None)
+    assert len(oAe.blocks) == 0
+    B = (oc.Leg(U1, Ae[1].charges, oc.IN, "j"), oc.Leg(U1, rng.integers(-1, 2, 5), oc.OUT, "k"))
+    oB = oc.SymT.random_normal(B, (2,), target=None)
+    assert len(oB.blocks) > 0
+    A_e, B_d = from_oracle(oAe), from_oracle(oB)
+    assert A_e.n_blocks == 0 and A_e.data.numel() == 0
+
+    oC, _ = ocn.contract_symmetric([oAe, oB], "ij,jk->ik")
+    C = contract(A_e, B_d)
+    assert_same_structure(C, oC)
+    assert C.n_blocks == len(oC.blocks) == 0 and C.labels() == ("i", "k")
+    assert torch.count_nonzero(C.todense()).item() == 0 and tuple(C.todense().shape) == (3, 5)
+
+    # operands with blocks whose contracted sectors never meet: X holds only
+    # the b = 1 sector, Y only b = 2
+    X = (oc.Leg(U1, np.array([1, 1]), oc.IN, "a"), oc.Leg(U1, np.array([1, 2, 2]), oc.OUT, "b"))
+    Y = (oc.Leg(U1, np.array([1, 2, 2]), oc.IN, "b"), oc.Leg(U1, np.array([2, 2, 2]), oc.OUT, "c"))
+    oX, oY = oc.SymT.random_normal(X, (3,), target=None), oc.SymT.random_normal(Y, (4,), target=None)
+    assert len(oX.blocks) > 0 and len(oY.blocks) > 0
+    oZ, _ = ocn.contract_symmetric([oX, oY], "ab,bc->ac")
+    Z = contract(from_oracle(oX), from_oracle(oY))
+    assert_same_structure(Z, oZ)
+    assert Z.n_blocks == len(oZ.blocks) == 0 and Z.labels() == ("a", "c")
+
+    # transpose and vector algebra on the empty tensor
+    T = A_e.transpose((1, 0))
+    assert T.n_blocks == 0 and T.labels() == ("j", "i")
+    assert float(A_e.norm()) == 0.0 and float(A_e.max_abs()) == 0.0
+    assert float(inner(A_e, A_e)) == 0.0
+    D = A_e + A_e
+    assert D.n_blocks == 0 and D.data.numel() == 0
