@@ -154,3 +154,37 @@ def test_copy2d_batched_rectangles(gpu):
         at += rows * dld + 3
     _lib.copy2d(copies, dst.device)
     assert torch.equal(dst, ref)
+
+
+@pytest.mark.parametrize("absorb", [None, "right", "left"])
+def test_tied_values_across_sectors_and_keep_all(gpu, absorb, factor_path):
+    """Two sectors holding the same block have tied singular values; the
+    global sort is stable (linalg.py:211), so the earlier sector's value comes
+    first and a cut inside a tie keeps it, and the new bond's charges follow.
+    A bound above the rank keeps every value.  Checked against the oracle on
+    the direct and the Gram path."""
+    from oracle import core as oc
+    from tenax_amd.linalg import truncated_svd_symmetric
+
+    U1 = oc.U1()
+    rng = np.random.default_rng(12)
+    rows = oc.Leg(U1, np.array([0] * 4 + [1] * 4 + [2] * 3), oc.IN, "r")
+    cols = oc.Leg(U1, np.array([0] * 5 + [1] * 5 + [2] * 4), oc.OUT, "c")
+    m0 = rng.standard_normal((4, 5))
+    T = oc.SymT({(0, 0): m0, (1, 1): m0.copy(), (2, 2): 0.01 * rng.standard_normal((3, 4))}, (rows, cols))
+    Td = from_oracle(T)
+    for chi in (1, 3, 5, 9, 100):
+        Uo, so, Vo, sfo, _ = ola.truncated_svd_symmetric(T, ["r"], ["c"], "bond", chi)
+        U, s, Vh, sf = truncated_svd_symmetric(Td, ["r"], ["c"], "bond", chi, absorb=absorb)
+        s = s.cpu().numpy()
+        assert len(s) == len(so) == min(chi, 11), chi
+        assert np.abs(s - so).max() < 1e-12 * so[0], chi
+        assert np.array_equal(U.indices[-1].charges, Uo.legs[-1].charges), (chi, U.indices[-1].charges)
+        ref = np.tensordot(Uo.todense() * so, Vo.todense(), axes=([1], [0]))
+        got = _dense_product(U, s, Vh, absorb)
+        assert np.abs(got - ref).max() < 1e-11 * so[0], chi
+        assert _isometry(U, Vh, absorb) < 1e-13, chi
+    # the ties themselves: sector 0 and sector 1 give bitwise-equal spectra
+    _, s, _, _ = truncated_svd_symmetric(Td, ["r"], ["c"], "bond", 100, absorb=absorb)
+    s = s.cpu().numpy()
+    assert np.array_equal(s[0:8:2], s[1:8:2])
