@@ -12,7 +12,8 @@ from tests.helpers import assert_same_structure, from_oracle, max_rel_err, to_or
 
 pytestmark = pytest.mark.gpu
 
-ED = {(4, 0): -1.616025403784438, (6, 0): -2.493577133887925, (4, 2): -0.957106781186548}
+ED = {(4, 0): -1.616025403784438, (6, 0): -2.493577133887925, (4, 2): -0.957106781186548,
+      (3, 1): -1.0, (5, 1): -1.9278862533179948}  # odd chains: the 2Sz = 1 sector
 
 
 def _oracle_state(L, chi, seed, sweeps=1):
@@ -289,7 +290,7 @@ def test_lanczos_graph_replays_match_eager(gpu, state8, reorth):
         prob.plan.set_tiling(3, 0, 0, 0)
 
 
-@pytest.mark.parametrize("L,tgt", [(4, 0), (6, 0), (4, 2)])
+@pytest.mark.parametrize("L,tgt", [(4, 0), (6, 0), (4, 2), (3, 1), (5, 1)])
 def test_dmrg_matches_ed(gpu, L, tgt):
     from tenax_amd import DMRGConfig, build_mpo_heisenberg, build_random_symmetric_mps, dmrg
 
