@@ -682,3 +682,41 @@ def test_empty_operands_and_results(gpu):
     assert float(inner(A_e, A_e)) == 0.0
     D = A_e + A_e
     assert D.n_blocks == 0 and D.data.numel() == 0
+
+
+def test_large_and_sparse_u1_charges(gpu):
+    """U(1) charges far from zero and far apart (sectors -100000, 0, 100000;
+    sums reaching 2e5 on a fused output and a target of 2e5): the device planner's valid-block
+    enumeration and key matching use the charge values themselves, so the
+    result must carry the oracle's keys, shapes and values
+    (contractor.py:291-484; core/index.py charge arithmetic on int32)."""
+    from tenax_amd import contract
+
+    U1 = oc.U1()
+    rng = np.random.default_rng(21)
+    qs = np.array([-100000, 0, 100000], np.int32)
+    a = oc.Leg(U1, rng.choice(qs, 12), oc.IN, "a")
+    b = oc.Leg(U1, rng.choice(qs, 10), oc.OUT, "b")
+    k = oc.Leg(U1, rng.choice(qs, 11), oc.IN, "k")
+    c = oc.Leg(U1, rng.choice(qs, 9), oc.OUT, "c")
+    A = (a, b, oc.Leg(U1, k.charges, oc.OUT, "k"))
+    B = (k, c)
+    oA = oc.SymT.random_normal(A, (50,), target=None)
+    oB = oc.SymT.random_normal(B, (51,), target=None)
+    assert len(oA.blocks) > 0 and len(oB.blocks) > 0
+    oC, _ = ocn.contract_symmetric([oA, oB], "abk,kc->abc")
+    C = contract(from_oracle(oA), from_oracle(oB))
+    assert_same_structure(C, oC)
+    assert len(oC.blocks) > 0 and max_rel_err(C, oC) < 1e-13
+    # a nonzero target charge far from zero
+    oT = oc.SymT.random_normal((a, b), (52,), target=200000)
+    assert len(oT.blocks) > 0
+    T = from_oracle(oT)
+    assert_same_structure(T, oT)
+    assert max_rel_err(T.transpose((1, 0)).transpose((1, 0)), oT) == 0.0
+    # the device enumeration of valid blocks at these charges and targets
+    from tenax_amd.core.tensor import valid_blocks
+
+    for legs, tgt in [((a, b, A[2]), None), ((a, b), 200000), ((a, b, A[2]), -100000)]:
+        keys, shapes = valid_blocks(product_indices(legs), tgt)
+        assert [tuple(int(q) for q in kk) for kk in keys] == sorted(oc.compute_valid_blocks(legs, tgt)), tgt
